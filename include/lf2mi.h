@@ -1,0 +1,225 @@
+/*
+ * lf2mi.h -- C ABI of the MI355X-native LATfield2 particle-mesh path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (LF2MI_OK = 0, negative on error; lf2mi_last_error() gives the
+ * message) and is stream-ordered: `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream).  Field and k-field pointers are DEVICE pointers.
+ * Host pointers appear only in the explicit upload/download calls.
+ *
+ * Geometry (box length 1, cubic lattice of N sites, resolution 1/N):
+ *   real field, C components, ghost width h, G = N + 2h  (Field<Real>):
+ *     data[((x+h) + G*((y+h) + G*(z+h)))*C + c]
+ *       -- LATfield2_Lattice.hpp:111-113 (strides), LATfield2_Field_decl.hpp:760-807
+ *   k-field of the R2C transform, ghost width kh, R = N/2+1+2kh, KG = N+2kh
+ *   (Field<Imag> on Lattice::initializeRealFFT, dims (kx, kz, ky)):
+ *     k[((kx+kh) + R*((kz+kh) + KG*(ky+kh)))*C + c]   (complex = {re, im})
+ *       -- LATfield2_PlanFFT_decl.hpp:1469-1510, LATfield2_Lattice.hpp:225-259
+ *   Symmetric 3x3 fields use C = 6 in the order (0,0)(0,1)(0,2)(1,1)(1,2)(2,2)
+ *       -- LATfield2_Field_decl.hpp:773-788
+ * Precision is a run-time argument: LF2MI_F32 (float / float2) or LF2MI_F64.
+ * The library is not thread-safe; use one host thread per device.
+ */
+#ifndef LF2MI_H
+#define LF2MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LF2MI_OK 0
+#define LF2MI_EINVAL (-1)      /* bad argument */
+#define LF2MI_ENOMEM (-2)      /* device or host allocation failed */
+#define LF2MI_EHIP (-3)        /* HIP runtime error */
+#define LF2MI_ERANGE (-4)      /* size outside the supported range */
+#define LF2MI_EUNSUPPORTED (-5)
+
+/* precision */
+#define LF2MI_F32 0
+#define LF2MI_F64 1
+
+/* FFT directions -- LATfield2_PlanFFT.hpp:14-15 */
+#define LF2MI_FFT_FORWARD 1
+#define LF2MI_FFT_BACKWARD (-1)
+
+/* projection kinds */
+#define LF2MI_PROJ_SCALAR 0 /* scalarProjectionCIC_*  (C = 1) */
+#define LF2MI_PROJ_VECTOR 1 /* vectorProjectionCIC_*  (C = 3) */
+#define LF2MI_PROJ_VECVEC 2 /* VecVecProjectionCIC_*  (C = 6) */
+
+/* deposit target */
+#define LF2MI_DEP_UNFOLDED 0 /* like *_project: adds into interior and ghost nodes */
+#define LF2MI_DEP_PERIODIC 1 /* like *_project then *_comm: adds into interior nodes */
+
+/* deposit accumulation */
+#define LF2MI_ACC_NATIVE 0  /* atomics in the container's precision */
+#define LF2MI_ACC_FIXED64 1 /* int64 fixed point, scale 2^-32: run-to-run reproducible */
+
+/* particle container flags */
+#define LF2MI_PARTS_ID 1   /* keep 64-bit particle IDs (part_simple::ID) */
+#define LF2MI_PARTS_MASS 2 /* per-particle mass (FROM_PART) instead of one global mass */
+
+/* ------------------------------------------------------------------ library */
+int lf2mi_version(void);
+/* message of the last failing call on this host thread ("" if none) */
+const char *lf2mi_last_error(void);
+/* select and initialise a device for the calling thread */
+int lf2mi_device_init(int device);
+/* hipDeviceSynchronize with status */
+int lf2mi_synchronize(void);
+
+/* Per-kernel timing (no counterpart in the reference, whose timer.hpp:64-177 and the
+ * MPI_Wtime brackets of benchmarks_tests/benchmarks_particles.cpp:189-236 time stages
+ * on the host).  When enabled, every kernel launch records a pair of HIP events on the
+ * stream it is launched on; lf2mi_profile_get synchronises on pending events and
+ * returns, for slot `index`, the kernel name, the summed duration and the launch count
+ * (LF2MI_ERANGE past the last slot).  lf2mi_profile_reset clears all slots. */
+int lf2mi_profile_enable(int on);
+int lf2mi_profile_reset(void);
+int lf2mi_profile_get(int index, char *name, int name_cap, double *total_ms, int64_t *count);
+
+/* ------------------------------------------------------------------ fields */
+/* projection_init: zero `count` values (particles/projections.hpp:31-34) */
+int lf2mi_zero(void *ptr, int64_t count, int dtype, void *stream);
+
+/* Field::updateHalo: periodic ghost fill of all C components
+ * (LATfield2_Field_decl.hpp:889-935); bit-exact copies */
+int lf2mi_ghost_fill(void *field, int N, int halo, int C, int dtype, void *stream);
+
+/* the ghost folds after a deposit, in the reference's order (x, then y, then z):
+ *   LF2MI_PROJ_SCALAR: scalarProjectionCIC_comm (particles/projections.hpp:194-300)
+ *   LF2MI_PROJ_VECTOR: vectorProjectionCIC_comm (particles/projections.hpp:1192-1351)
+ *   LF2MI_PROJ_VECVEC: VecVecProjectionCIC_comm (particles/projections.hpp:1624-1814) */
+int lf2mi_projection_comm(void *field, int N, int halo, int kind, int dtype, void *stream);
+
+/* sum over the interior of component `comp` in FP64 (parallel.sum of the driver loop in
+ * examples/poissonSolver.cpp:88-94); *out is written on the host, the call synchronises */
+int lf2mi_field_sum(const void *field, int N, int halo, int C, int comp, int dtype,
+                    double *out, void *stream);
+
+/* field[interior, comp] = field * mul + add  (driver-side arithmetic of the Poisson
+ * example: mean subtraction, 1/N^3 normalisation) */
+int lf2mi_field_affine(void *field, int N, int halo, int C, int comp, double mul, double add,
+                       int dtype, void *stream);
+
+/* ------------------------------------------------------------------ FFT */
+typedef struct lf2mi_fft_plan_s *lf2mi_fft_plan;
+
+/* PlanFFT<Imag>(Field<Real>*, Field<Imag>*) for the R2C case
+ * (LATfield2_PlanFFT_decl.hpp:731-846): N a power of two in [8, 1024], C >= 1
+ * components, real ghost width rhalo, k ghost width khalo.  Owns its twiddle tables
+ * and workspace on the current device. */
+int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan);
+int lf2mi_fft_plan_destroy(lf2mi_fft_plan plan);
+/* bytes of device memory the plan holds */
+int lf2mi_fft_plan_bytes(lf2mi_fft_plan plan, int64_t *bytes);
+
+/* PlanFFT::execute (LATfield2_PlanFFT_decl.hpp:851-1056).  FORWARD: unnormalised
+ * e^{-2 pi i k.x/N} from the real interior into the k-field interior (input kept).
+ * BACKWARD: unnormalised e^{+}, k-field kept, real interior written (ghosts untouched);
+ * only the real parts of the kx = 0 and kx = N/2 modes are used, as in FFTW's c2r.
+ * Any other direction is a no-op, as in the reference. */
+int lf2mi_fft_execute(lf2mi_fft_plan plan, int direction, void *rfield, void *kfield, void *stream);
+
+/* Green's-function multiply of examples/poissonSolver.cpp:99-111 on a k-field:
+ * k *= scale / lambda, lambda = 2 (cos 2pi kx/N + cos 2pi ky/N + cos 2pi kz/N - 3) / res^2,
+ * zero mode set to 0; lambda and scale/lambda evaluated in FP64. */
+int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, double scale,
+                        int dtype, void *stream);
+
+/* Fused Poisson solve on the plan's geometry (C must be 1):
+ *   phi = C2R( scale * G(k) * R2C(rho - shift) ),  G = -1/K^2, G(0) = 0,
+ *   K^2 = sum_i N^2 (2 - 2 cos(2 pi k_i / N))
+ * i.e. the driver sequence mean-subtract -> execute(FORWARD) -> Green -> execute(BACKWARD)
+ * with the 1/N^3 normalisation folded into `scale` by the caller.  rho is read only;
+ * the plan workspace holds the k-space data.  No k-field is materialised. */
+int lf2mi_poisson_solve(lf2mi_fft_plan plan, const void *rho, void *phi, double shift,
+                        double scale, void *stream);
+
+/* ------------------------------------------------------------------ particles */
+/* Particles<part_simple, part_simple_info, ...> (particles/LATfield2_Particles.hpp:131-344)
+ * on one device.  LF2MI_F32: each particle is kept as an int32 cell index
+ * x + N(y + N z) plus an FP32 offset in cell units in [0,1) per axis and FP32
+ * velocities, bucketed by 8^3-cell bricks (the analogue of the per-cell lists).
+ * LF2MI_F64: absolute FP64 positions and velocities in upload order (plain path). */
+typedef struct lf2mi_parts_s *lf2mi_parts;
+
+int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_parts *out);
+int lf2mi_parts_destroy(lf2mi_parts p);
+/* part_info mass (FROM_INFO), default 1 */
+int lf2mi_parts_set_mass(lf2mi_parts p, double mass_global);
+int lf2mi_parts_count(lf2mi_parts p, int64_t *np);
+int lf2mi_parts_bytes(lf2mi_parts p, int64_t *bytes);
+
+/* addParticle_global for np particles (particles/LATfield2_Particles.hpp:501-519):
+ * host arrays pos[3*np] (box units), vel[3*np], optional mass[np] and id[np].  Cells
+ * are (int)floor(pos*N) % N per axis computed on the device in FP64 (:464-467);
+ * particles with a negative coordinate are dropped as the reference drops them.
+ * Replaces the container's contents.  *kept (may be NULL) = particles stored.  Syncs. */
+int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const double *vel,
+                       const double *mass, const int64_t *id, int64_t *kept, void *stream);
+
+/* copy the particles to the host in container order: pos as (cell + offset)/N for F32
+ * (exactly the positions the container represents), vel, mass, id, cell (any may be
+ * NULL).  Synchronises. */
+int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, int64_t *id,
+                         int32_t *cell, void *stream);
+
+/* per-cell particle counts into a device int32 array of N^3 entries (bit-exact with
+ * the histogram of Particles::getPartCoord keys) */
+int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream);
+
+/* synthetic initial conditions generated on the device (counter-based RNG keyed by
+ * seed and particle index; positions produced in FP64 and binned as in upload):
+ *   lattice: ng^3 particles on i/ng plus a Gaussian displacement of sigma_cells/N per
+ *            axis, Gaussian velocities of sigma_v;
+ *   uniform: np uniform positions, Gaussian velocities of sigma_v;
+ *   blobs  : (1-frac_uniform)*np particles in nblobs isotropic Gaussian blobs of
+ *            sigma_cells/N around uniform centres with Pareto(alpha) occupancies
+ *            (largest-remainder rounding), the rest uniform; velocities sigma_v.
+ * IDs (if kept) are the generation index.  Synchronises. */
+int lf2mi_parts_ic_lattice(lf2mi_parts p, int ng, double sigma_cells, double sigma_v,
+                           uint64_t seed, void *stream);
+int lf2mi_parts_ic_uniform(lf2mi_parts p, int64_t np, double sigma_v, uint64_t seed,
+                           void *stream);
+int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha,
+                         double sigma_cells, double frac_uniform, double sigma_v,
+                         uint64_t seed, void *stream);
+
+/* CIC deposit of the container into `field` (device, the container's precision):
+ *   kind  LF2MI_PROJ_SCALAR -> scalarProjectionCIC_project  (projections.hpp:76-184)
+ *         LF2MI_PROJ_VECTOR -> vectorProjectionCIC_project  (projections.hpp:980-1189)
+ *         LF2MI_PROJ_VECVEC -> VecVecProjectionCIC_project  (projections.hpp:1356-1622)
+ *   target LF2MI_DEP_UNFOLDED adds exactly what the reference _project adds (ghost
+ *          nodes included, follow with lf2mi_projection_comm); LF2MI_DEP_PERIODIC adds
+ *          the folded result into the interior (== _project + _comm on the interior).
+ *   acc   LF2MI_ACC_NATIVE or LF2MI_ACC_FIXED64 (F32 scalar only).
+ * Values are added (+=): zero the field first, as projection_init does. */
+int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc,
+                  void *stream);
+
+/* Fused CIC gather + kick + drift (Particles::updateVel then moveParticles with the
+ * gather functor, particles/LATfield2_Particles.hpp:753-893, :1811-2803):
+ *   F = -grad phi by central differences (phi(n+e) - phi(n-e)) N/2 with periodic wrap
+ *   (reads the interior of phi only); vel += dt_kick * sum_corners w F(corner) with the
+ *   scalar-deposit weights; pos += dt_drift * vel, wrapped (:2028-2032) and re-keyed.
+ * In F32 the new position is formed in FP64 from (cell, offset, vel) and re-anchored to
+ * (cell, FP32 offset).  The container is re-bucketed afterwards.  vmax2 (host, may be
+ * NULL) receives max |v|^2 after the kick (synchronises when non-NULL). */
+int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick,
+                            double dt_drift, double *vmax2, void *stream);
+
+/* One PM step (the driver sequence of examples/poissonSolver.cpp plus the particle
+ * update): zero rho, periodic CIC deposit, mean-subtracted Poisson solve with
+ * G = -1/K^2 and 1/N^3, gather + kick + drift, re-bucket.  rho and phi are real fields
+ * with ghost width `halo` in the container's precision; plan must match (N, C=1). */
+int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int halo,
+                  double dt_kick, double dt_drift, int acc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+"""latfield2_amd -- MI355X-native particle-mesh path of LATfield2.
+
+Python mirror of the reference's host interface for the PM path, driving the
+HIP kernels of ``liblf2mi.so`` through the C ABI of ``include/lf2mi.h``.  torch
+supplies device memory and the current stream; it does no arithmetic here.
+
+Names follow the reference (paths relative to the LATfield2 tree):
+  Lattice              LATfield2_Lattice_decl.hpp:13-218
+  Field                LATfield2_Field_decl.hpp:61-430 (AoS storage, updateHalo)
+  KField               Field<Imag> on Lattice::initializeRealFFT (dims kx, kz, ky)
+  PlanFFT              LATfield2_PlanFFT_decl.hpp:83-1577 (R2C), FFT_FORWARD/BACKWARD
+  Particles            particles/LATfield2_Particles.hpp:131-344
+  projection_init, scalarProjectionCIC_project/_comm, vectorProjectionCIC_project/_comm,
+  VecVecProjectionCIC_project/_comm   particles/projections.hpp
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64,
+                   FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_VECTOR,
+                   PROJ_VECVEC, Lf2miError, call, load)
+
+__all__ = [
+    "Lattice", "Field", "KField", "PlanFFT", "Particles", "projection_init",
+    "scalarProjectionCIC_project", "scalarProjectionCIC_comm", "vectorProjectionCIC_project",
+    "vectorProjectionCIC_comm", "VecVecProjectionCIC_project", "VecVecProjectionCIC_comm",
+    "poisson_green", "poisson_solve", "pm_step", "FFT_FORWARD", "FFT_BACKWARD", "F32", "F64",
+    "Lf2miError", "load",
+]
+
+FROM_INFO = 0  # particles/projections.hpp:39-40 (global mass)
+FROM_PART = 1  # per-particle mass
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dtype_code(dtype) -> int:
+    torch = _torch()
+    if dtype in (torch.float32, np.float32, "f32", F32):
+        return F32
+    if dtype in (torch.float64, np.float64, "f64", F64):
+        return F64
+    raise ValueError(f"unsupported precision {dtype}")
+
+
+def _torch_dtype(code: int):
+    torch = _torch()
+    return torch.float32 if code == F32 else torch.float64
+
+
+class Lattice:
+    """Cubic 3-D lattice of N sites per axis with ghost width `halo` (box length 1)."""
+
+    def __init__(self, dim: int, N: int, halo: int):
+        if dim != 3:
+            raise ValueError("the PM path is three-dimensional")
+        self.dim, self.N, self.halo = dim, int(N), int(halo)
+
+    def size(self, i: int = 0) -> int:
+        return self.N
+
+    def sizeLocal(self, i: int = 0) -> int:
+        return self.N
+
+    def jump(self, i: int) -> int:
+        return (self.N + 2 * self.halo) ** i
+
+    def sitesLocalGross(self) -> int:
+        return (self.N + 2 * self.halo) ** 3
+
+    def sites(self) -> int:
+        return self.N ** 3
+
+
+class Field:
+    """Field<Real> with C components on the device: storage [z, y, x, c] with ghosts."""
+
+    def __init__(self, lat: Lattice, components: int = 1, dtype=F32, device: str = "cuda"):
+        torch = _torch()
+        self.lat, self.C, self.code = lat, int(components), _dtype_code(dtype)
+        G = lat.N + 2 * lat.halo
+        self.t = torch.zeros((G, G, G, self.C), dtype=_torch_dtype(self.code), device=device)
+
+    @property
+    def N(self):
+        return self.lat.N
+
+    @property
+    def halo(self):
+        return self.lat.halo
+
+    def components(self) -> int:
+        return self.C
+
+    def data(self):
+        return self.t
+
+    def interior(self):
+        h = self.halo
+        G = self.N + 2 * h
+        return self.t[h:G - h, h:G - h, h:G - h, :]
+
+    def updateHalo(self):
+        call("lf2mi_ghost_fill", _ptr(self.t), self.N, self.halo, self.C, self.code, _stream())
+
+    def zero(self):
+        call("lf2mi_zero", _ptr(self.t), self.t.numel(), self.code, _stream())
+
+    def sum(self, comp: int = 0) -> float:
+        out = ctypes.c_double(0.0)
+        call("lf2mi_field_sum", _ptr(self.t), self.N, self.halo, self.C, comp, self.code,
+             ctypes.byref(out), _stream())
+        return out.value
+
+    def affine(self, mul: float, add: float, comp: int = 0):
+        call("lf2mi_field_affine", _ptr(self.t), self.N, self.halo, self.C, comp, float(mul),
+             float(add), self.code, _stream())
+
+
+class KField:
+    """Field<Imag> on the R2C k-lattice: storage [ky, kz, kx, c, re/im] with khalo ghosts."""
+
+    def __init__(self, lat: Lattice, khalo: int = 0, components: int = 1, dtype=F32, device: str = "cuda"):
+        torch = _torch()
+        self.N, self.kh, self.C, self.code = lat.N, int(khalo), int(components), _dtype_code(dtype)
+        KG = self.N + 2 * self.kh
+        R = self.N // 2 + 1 + 2 * self.kh
+        self.t = torch.zeros((KG, KG, R, self.C, 2), dtype=_torch_dtype(self.code), device=device)
+
+    def data(self):
+        return self.t
+
+    def interior(self):
+        kh, N = self.kh, self.N
+        return self.t[kh:kh + N, kh:kh + N, kh:kh + N // 2 + 1]
+
+    def as_complex(self):
+        return _torch().view_as_complex(self.t)
+
+
+class PlanFFT:
+    """PlanFFT<Imag>(Field<Real>*, Field<Imag>*) -- R2C/C2R on the device."""
+
+    def __init__(self, rfield: Field, kfield: KField | None = None):
+        if kfield is not None and (kfield.N != rfield.N or kfield.C != rfield.C or kfield.code != rfield.code):
+            raise ValueError("PlanFFT: real and k fields disagree on size, components or precision")
+        self.r, self.k = rfield, kfield
+        self.N, self.C, self.code = rfield.N, rfield.C, rfield.code
+        kh = kfield.kh if kfield is not None else 0
+        h = ctypes.c_void_p()
+        call("lf2mi_fft_plan_create", self.N, self.C, rfield.halo, kh, self.code, ctypes.byref(h))
+        self.h = h
+
+    def execute(self, fft_type: int):
+        call("lf2mi_fft_execute", self.h, int(fft_type), _ptr(self.r.t), _ptr(self.k.t), _stream())
+
+    def bytes(self) -> int:
+        b = ctypes.c_int64(0)
+        call("lf2mi_fft_plan_bytes", self.h, ctypes.byref(b))
+        return b.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and _lib._lib is not None:
+                _lib._lib.lf2mi_fft_plan_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def poisson_green(k: KField, res: float | None = None, scale: float = 1.0):
+    """examples/poissonSolver.cpp:99-111 on the device (lambda and scale/lambda in FP64)."""
+    res = 1.0 / k.N if res is None else res
+    call("lf2mi_poisson_green", _ptr(k.t), k.N, k.C, k.kh, float(res), float(scale), k.code, _stream())
+
+
+def poisson_solve(plan: PlanFFT, rho: Field, phi: Field, shift: float, scale: float):
+    """Fused phi = C2R(scale * (-1/K^2) * R2C(rho - shift)) with the zero mode removed."""
+    call("lf2mi_poisson_solve", plan.h, _ptr(rho.t), _ptr(phi.t), float(shift), float(scale), _stream())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Particles:
+    """Particles<part_simple, part_simple_info, ...> on the device (box length 1)."""
+
+    def __init__(self, N: int, capacity: int, dtype=F32, ids: bool = False, per_particle_mass: bool = False,
+                 mass: float = 1.0):
+        self.N, self.code = int(N), _dtype_code(dtype)
+        flags = (PARTS_ID if ids else 0) | (PARTS_MASS if per_particle_mass else 0)
+        self.flags = flags
+        h = ctypes.c_void_p()
+        call("lf2mi_parts_create", self.N, int(capacity), self.code, flags, ctypes.byref(h))
+        self.h = h
+        call("lf2mi_parts_set_mass", self.h, float(mass))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and _lib._lib is not None:
+                _lib._lib.lf2mi_parts_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def count(self) -> int:
+        n = ctypes.c_int64(0)
+        call("lf2mi_parts_count", self.h, ctypes.byref(n))
+        return n.value
+
+    def bytes(self) -> int:
+        n = ctypes.c_int64(0)
+        call("lf2mi_parts_bytes", self.h, ctypes.byref(n))
+        return n.value
+
+    def addParticles(self, pos, vel, mass=None, ids=None) -> int:
+        """addParticle_global for arrays of particles; returns the number kept."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64)
+        kept = ctypes.c_int64(0)
+        call("lf2mi_parts_upload", self.h, pos.shape[0], _dp(pos), _dp(vel), _dp(m),
+             None if i is None else i.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(kept),
+             _stream())
+        return kept.value
+
+    def download(self):
+        """dict of numpy arrays in container order: pos (box units), vel, mass, cell, id (if kept)"""
+        n = self.count()
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        mass = np.zeros(n)
+        cell = np.zeros(n, dtype=np.int32)
+        ids = np.zeros(n, dtype=np.int64) if self.flags & PARTS_ID else None
+        call("lf2mi_parts_download", self.h, _dp(pos), _dp(vel), _dp(mass),
+             None if ids is None else ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+             cell.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _stream())
+        out = {"pos": pos, "vel": vel, "mass": mass, "cell": cell}
+        if ids is not None:
+            out["id"] = ids
+        return out
+
+    def cell_counts(self):
+        torch = _torch()
+        c = torch.empty(self.N ** 3, dtype=torch.int32, device="cuda")
+        call("lf2mi_parts_cell_counts", self.h, _ptr(c), _stream())
+        return c
+
+    def ic_lattice(self, ng: int, sigma_cells: float, sigma_v: float, seed: int):
+        call("lf2mi_parts_ic_lattice", self.h, int(ng), float(sigma_cells), float(sigma_v), int(seed), _stream())
+
+    def ic_uniform(self, np_: int, sigma_v: float, seed: int):
+        call("lf2mi_parts_ic_uniform", self.h, int(np_), float(sigma_v), int(seed), _stream())
+
+    def ic_blobs(self, np_: int, nblobs: int = 4096, alpha: float = 1.5, sigma_cells: float = 2.0,
+                 frac_uniform: float = 0.2, sigma_v: float = 0.0, seed: int = 1):
+        call("lf2mi_parts_ic_blobs", self.h, int(np_), int(nblobs), float(alpha), float(sigma_cells),
+             float(frac_uniform), float(sigma_v), int(seed), _stream())
+
+    def deposit(self, kind: int, field: Field, target: int = DEP_UNFOLDED, acc: int = ACC_NATIVE):
+        call("lf2mi_deposit", self.h, int(kind), _ptr(field.t), field.halo, int(target), int(acc), _stream())
+
+    def gather_kick_drift(self, phi: Field, dt_kick: float, dt_drift: float, want_vmax: bool = False):
+        vm = ctypes.c_double(0.0)
+        call("lf2mi_gather_kick_drift", self.h, _ptr(phi.t), phi.halo, float(dt_kick), float(dt_drift),
+             ctypes.byref(vm) if want_vmax else None, _stream())
+        return vm.value if want_vmax else None
+
+
+def projection_init(f: Field):
+    """particles/projections.hpp:31-34"""
+    f.zero()
+
+
+def scalarProjectionCIC_project(parts: Particles, rho: Field):
+    parts.deposit(PROJ_SCALAR, rho, DEP_UNFOLDED)
+
+
+def scalarProjectionCIC_comm(rho: Field):
+    call("lf2mi_projection_comm", _ptr(rho.t), rho.N, rho.halo, PROJ_SCALAR, rho.code, _stream())
+
+
+def vectorProjectionCIC_project(parts: Particles, B: Field):
+    parts.deposit(PROJ_VECTOR, B, DEP_UNFOLDED)
+
+
+def vectorProjectionCIC_comm(B: Field):
+    call("lf2mi_projection_comm", _ptr(B.t), B.N, B.halo, PROJ_VECTOR, B.code, _stream())
+
+
+def VecVecProjectionCIC_project(parts: Particles, T: Field):
+    parts.deposit(PROJ_VECVEC, T, DEP_UNFOLDED)
+
+
+def VecVecProjectionCIC_comm(T: Field):
+    call("lf2mi_projection_comm", _ptr(T.t), T.N, T.halo, PROJ_VECVEC, T.code, _stream())
+
+
+def pm_step(parts: Particles, plan: PlanFFT, rho: Field, phi: Field, dt_kick: float, dt_drift: float,
+            acc: int = ACC_NATIVE):
+    call("lf2mi_pm_step", parts.h, plan.h, _ptr(rho.t), _ptr(phi.t), rho.halo, float(dt_kick), float(dt_drift),
+         int(acc), _stream())

@@ -1,0 +1,708 @@
+// 3-D R2C / C2R FFT of the lf2mi library (PlanFFT<Imag> for the R2C case,
+// LATfield2_PlanFFT_decl.hpp:731-1056), written for gfx950.
+//
+// Layout contract (same as the reference k-field): k[((kx+kh) + R((kz+kh) + KG(ky+kh)))C + c]
+// with R = N/2+1+2kh, KG = N+2kh.  Rows of the k-field are indexed by two "slots":
+// slot1 = lattice dim 1 (kz), slot2 = lattice dim 2 (ky).
+//
+// Forward (in place in the k-field, no workspace):
+//   x pass : two real rows (y, y+1 of plane z) as one complex FFT of length N, split into
+//            the two half spectra; row (y, z) lands at slot1 = z, slot2 = y
+//   z pass : complex FFT along slot1 (z -> kz), in place
+//   y pass : complex FFT along slot2 (y -> ky), in place          => (kx, kz, ky)
+// Backward (k-field preserved, plan workspace W with the same row structure):
+//   y pass : slot2 ky -> y, k-field -> W;  z pass : slot1 kz -> z, in place on W;
+//   x pass : Hermitian rows (real parts of kx = 0, N/2) -> two real rows per FFT.
+// Each pass moves every element once through HBM; all butterflies run in LDS
+// (Stockham auto-sort, radix 8/4), twiddles come from an FP64-generated table.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <new>
+
+#include "common.hpp"
+
+namespace lf2mi {
+
+// ---------------------------------------------------------------- small DFTs
+template <int S, typename V> __device__ __forceinline__ V mul_si(V a) {  // a * (S i)
+    V r;
+    if (S < 0) { r.x = a.y; r.y = -a.x; }
+    else { r.x = -a.y; r.y = a.x; }
+    return r;
+}
+
+template <int S, typename V> __device__ __forceinline__ void dft2(V &a0, V &a1) {
+    V t = a1;
+    a1 = csub(a0, t);
+    a0 = cadd(a0, t);
+}
+
+template <int S, typename V> __device__ __forceinline__ void dft4(V &a0, V &a1, V &a2, V &a3) {
+    V t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+template <int S, typename V> __device__ __forceinline__ void dft8(V *a) {
+    using T = decltype(a[0].x);
+    const T r2 = (T)0.70710678118654752440;
+    dft4<S>(a[0], a[2], a[4], a[6]);
+    dft4<S>(a[1], a[3], a[5], a[7]);
+    // odd terms times w8^k, w8 = exp(S 2 pi i / 8)
+    V o1, o2, o3;
+    o1.x = r2 * (a[3].x - S * a[3].y);
+    o1.y = r2 * (a[3].y + S * a[3].x);
+    o2 = mul_si<S>(a[5]);
+    o3.x = r2 * (-a[7].x - S * a[7].y);
+    o3.y = r2 * (-a[7].y + S * a[7].x);
+    V e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6], o0 = a[1];
+    a[0] = cadd(e0, o0);
+    a[4] = csub(e0, o0);
+    a[1] = cadd(e1, o1);
+    a[5] = csub(e1, o1);
+    a[2] = cadd(e2, o2);
+    a[6] = csub(e2, o2);
+    a[3] = cadd(e3, o3);
+    a[7] = csub(e3, o3);
+}
+
+template <int R, int S, typename V> __device__ __forceinline__ void dft(V *a) {
+    if constexpr (R == 2) dft2<S>(a[0], a[1]);
+    else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
+    else dft8<S>(a);
+}
+
+// ---------------------------------------------------------------- Stockham passes in LDS
+// Tile of L lines of length N; element e of line l lives at lds[e*ES + l].
+// One pass of radix R after sub-transforms of size NS: butterfly j reads e = j + k N/R,
+// multiplies by W_N^{(j mod NS) k N/(NS R)}, does a radix-R DFT and writes
+// e = (j/NS) NS R + (j mod NS) + k NS.
+template <typename T, int N, int L, int ES, int WG, int S, int R, int NS>
+__device__ __forceinline__ void stockham_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
+    using V = cplx<T>;
+    constexpr int NBF = L * (N / R);
+    constexpr int PER = (NBF + WG - 1) / WG;
+    V a[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int bf = threadIdx.x + b * WG;
+        if (NBF % WG == 0 || bf < NBF) {
+            const int l = bf % L, j = bf / L;
+#pragma unroll
+            for (int k = 0; k < R; ++k) a[b][k] = lds[(j + k * (N / R)) * ES + l];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int bf = threadIdx.x + b * WG;
+        if (NBF % WG == 0 || bf < NBF) {
+            const int l = bf % L, j = bf / L;
+            const int m = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int k = 1; k < R; ++k) {
+                    V w = tw[m * k * (N / (NS * R))];
+                    if (S < 0) w.y = -w.y;
+                    a[b][k] = cmul(a[b][k], w);
+                }
+            }
+            dft<R, S>(a[b]);
+            const int base = (j / NS) * NS * R + m;
+#pragma unroll
+            for (int k = 0; k < R; ++k) lds[(base + k * NS) * ES + l] = a[b][k];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int L, int ES, int WG, int S>
+__device__ __forceinline__ void tile_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
+#define LF2_PASS(R, NS) stockham_pass<T, N, L, ES, WG, S, R, NS>(lds, tw)
+    if constexpr (N == 8) { LF2_PASS(8, 1); }
+    else if constexpr (N == 16) { LF2_PASS(4, 1); LF2_PASS(4, 4); }
+    else if constexpr (N == 32) { LF2_PASS(8, 1); LF2_PASS(4, 8); }
+    else if constexpr (N == 64) { LF2_PASS(8, 1); LF2_PASS(8, 8); }
+    else if constexpr (N == 128) { LF2_PASS(8, 1); LF2_PASS(4, 8); LF2_PASS(4, 32); }
+    else if constexpr (N == 256) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(4, 64); }
+    else if constexpr (N == 512) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(8, 64); }
+    else if constexpr (N == 1024) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(4, 64); LF2_PASS(4, 256); }
+#undef LF2_PASS
+}
+
+// ---------------------------------------------------------------- kernel arguments
+template <typename T> struct StridedArgs {
+    const cplx<T> *in;
+    cplx<T> *out;
+    int64_t in_col0, out_col0;  // offset of column 0
+    int64_t in_es, out_es;      // stride between elements of a line (transform axis)
+    int64_t in_bs, out_bs;      // stride between batches (the other row axis)
+    int ncols;                  // valid columns per row
+    const cplx<T> *tw;
+    // Green fusion (MODE 1): element (col, e, batch) has kx = col / C, ky = e, kz = batch
+    const double *green_s;      // s[k] = N^2 (2 - 2 cos(2 pi k / N))
+    double green_scale;
+    int C;
+};
+
+template <typename T> struct XArgs {
+    const void *real_in;  // forward: real field (T); backward: unused
+    T *real_out;          // backward: real field
+    const cplx<T> *kin;   // backward: row source
+    cplx<T> *kout;        // forward: row destination
+    int C;
+    int rh;               // real ghost width
+    int64_t k_col0;       // offset of column (kx = 0, c = 0) of row (slot1 = 0, slot2 = 0)
+    int64_t k_s1, k_s2;   // strides of slot1 and slot2 (complex units)
+    T shift;              // forward: subtracted from every real value
+    T post;               // backward: multiplies every real output
+    const cplx<T> *tw;
+};
+
+// ---------------------------------------------------------------- strided passes (y, z)
+// grid: x = column tiles of L, y = batches.  MODE 0: one transform of sign S.
+// MODE 1: forward transform, Green multiply, inverse transform (fused Poisson y pass).
+template <typename T, int N, int L, int WG, int S, int MODE>
+__global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
+    using V = cplx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    const int col0 = blockIdx.x * L;
+    const int64_t batch = blockIdx.y;
+    const V *in = A.in + A.in_col0 + batch * A.in_bs + col0;
+    for (int i = threadIdx.x; i < N * L; i += WG) {
+        const int l = i % L, e = i / L;
+        V v = mk<T>(0, 0);
+        if (col0 + l < A.ncols) v = in[(int64_t)e * A.in_es + l];
+        lds[e * L + l] = v;
+    }
+    __syncthreads();
+    if constexpr (MODE == 0) {
+        tile_fft<T, N, L, L, WG, S>(lds, A.tw);
+    } else {
+        tile_fft<T, N, L, L, WG, -1>(lds, A.tw);
+        const double skz = A.green_s[batch];
+        for (int i = threadIdx.x; i < N * L; i += WG) {
+            const int l = i % L, e = i / L;
+            const int col = col0 + l;
+            if (col < A.ncols) {
+                const int kx = col / A.C;
+                const double k2 = A.green_s[kx] + A.green_s[e] + skz;
+                const T g = (k2 > 0.0) ? (T)(-A.green_scale / k2) : T(0);
+                V v = lds[e * L + l];
+                v.x *= g;
+                v.y *= g;
+                lds[e * L + l] = v;
+            }
+        }
+        __syncthreads();
+        tile_fft<T, N, L, L, WG, +1>(lds, A.tw);
+    }
+    V *out = A.out + A.out_col0 + batch * A.out_bs + col0;
+    for (int i = threadIdx.x; i < N * L; i += WG) {
+        const int l = i % L, e = i / L;
+        if (col0 + l < A.ncols) out[(int64_t)e * A.out_es + l] = lds[e * L + l];
+    }
+}
+
+// ---------------------------------------------------------------- x pass, R2C
+// Block = P row pairs x C components = P*C complex lines; line (p, c) carries rows
+// (2p, 2p+1) of component c as re/im.  Row r = y + N z of the real interior.
+template <typename T, int N, int L, int WG>
+__global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
+    using V = cplx<T>;
+    constexpr int ES = L + 1;
+    extern __shared__ unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    const int C = A.C;
+    const int P = L / C;
+    const int64_t pair0 = (int64_t)blockIdx.x * P;
+    const int64_t npairs = (int64_t)N * N / 2;
+    const int rh = A.rh;
+    const int64_t G = N + 2 * rh;
+    const T *real = (const T *)A.real_in;
+    const int NC = N * C;
+    // load: (p, ab, n, c) with (n, c) fastest -> contiguous within a row
+    for (int i = threadIdx.x; i < P * 2 * NC; i += WG) {
+        const int q = i % NC;
+        const int rest = i / NC;
+        const int ab = rest & 1, p = rest >> 1;
+        const int64_t pr = pair0 + p;
+        if (pr >= npairs) continue;
+        const int64_t row = 2 * pr + ab;
+        const int y = (int)(row % N), z = (int)(row / N);
+        const int n = q / C, c = q % C;
+        const int64_t src = (((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C) + q;
+        const T v = real[src] - A.shift;
+        T *slot = reinterpret_cast<T *>(&lds[n * ES + p * C + c]);
+        slot[ab] = v;
+    }
+    __syncthreads();
+    tile_fft<T, N, L, ES, WG, -1>(lds, A.tw);
+    // split the two half spectra: A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = -i (Z[k] - conj Z[N-k]) / 2
+    constexpr int H = N / 2 + 1;
+    const int HC = H * C;
+    for (int i = threadIdx.x; i < P * 2 * HC; i += WG) {
+        const int q = i % HC;
+        const int rest = i / HC;
+        const int ab = rest & 1, p = rest >> 1;
+        const int64_t pr = pair0 + p;
+        if (pr >= npairs) continue;
+        const int kx = q / C, c = q % C;
+        const int l = p * C + c;
+        const V zk = lds[kx * ES + l];
+        const V zm = lds[((N - kx) & (N - 1)) * ES + l];
+        V r;
+        if (ab == 0) {
+            r.x = (T)0.5 * (zk.x + zm.x);
+            r.y = (T)0.5 * (zk.y - zm.y);
+        } else {
+            r.x = (T)0.5 * (zk.y + zm.y);
+            r.y = (T)0.5 * (zm.x - zk.x);
+        }
+        const int64_t row = 2 * pr + ab;
+        const int y = (int)(row % N), z = (int)(row / N);
+        A.kout[A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)y * A.k_s2 + q] = r;
+    }
+}
+
+// ---------------------------------------------------------------- x pass, C2R
+template <typename T, int N, int L, int WG>
+__global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
+    using V = cplx<T>;
+    constexpr int ES = L + 1;
+    extern __shared__ unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    const int C = A.C;
+    const int P = L / C;
+    const int64_t pair0 = (int64_t)blockIdx.x * P;
+    const int64_t npairs = (int64_t)N * N / 2;
+    constexpr int H = N / 2 + 1;
+    const int HC = H * C;
+    // gather rows a, b into Z: Z[k] = A + iB (k in [0, N/2]), Z[N-k] = conj A + i conj B
+    for (int i = threadIdx.x; i < P * HC; i += WG) {
+        const int q = i % HC;
+        const int p = i / HC;
+        const int64_t pr = pair0 + p;
+        if (pr >= npairs) continue;
+        const int kx = q / C, c = q % C;
+        const int l = p * C + c;
+        const int64_t ra = 2 * pr, rb = ra + 1;
+        const int ya = (int)(ra % N), yb = (int)(rb % N), z = (int)(ra / N);
+        V a = A.kin[A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)ya * A.k_s2 + q];
+        V b = A.kin[A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)yb * A.k_s2 + q];
+        if (kx == 0 || kx == N / 2) {
+            a.y = 0;
+            b.y = 0;
+        }
+        V z1;
+        z1.x = a.x - b.y;
+        z1.y = a.y + b.x;
+        lds[kx * ES + l] = z1;
+        if (kx != 0 && kx != N / 2) {
+            V z2;
+            z2.x = a.x + b.y;
+            z2.y = b.x - a.y;
+            lds[(N - kx) * ES + l] = z2;
+        }
+    }
+    __syncthreads();
+    tile_fft<T, N, L, ES, WG, +1>(lds, A.tw);
+    const int rh = A.rh;
+    const int64_t G = N + 2 * rh;
+    const int NC = N * C;
+    for (int i = threadIdx.x; i < P * 2 * NC; i += WG) {
+        const int q = i % NC;
+        const int rest = i / NC;
+        const int ab = rest & 1, p = rest >> 1;
+        const int64_t pr = pair0 + p;
+        if (pr >= npairs) continue;
+        const int64_t row = 2 * pr + ab;
+        const int y = (int)(row % N), z = (int)(row / N);
+        const int n = q / C, c = q % C;
+        const V v = lds[n * ES + p * C + c];
+        const int64_t dst = (((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C) + q;
+        A.real_out[dst] = (ab ? v.y : v.x) * A.post;
+    }
+}
+
+// ---------------------------------------------------------------- launch configuration
+template <typename T> struct Tune;
+template <> struct Tune<float> {
+    static constexpr int WG = 256;
+    template <int N> static constexpr int strided_L() { return N >= 1024 ? 8 : 16; }
+    template <int N> static constexpr int x_L() { return N >= 512 ? 6 : (N >= 128 ? 12 : 24); }
+};
+template <> struct Tune<double> {
+    static constexpr int WG = 256;
+    template <int N> static constexpr int strided_L() { return N >= 1024 ? 4 : 8; }
+    template <int N> static constexpr int x_L() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
+};
+
+}  // namespace lf2mi
+
+// ---------------------------------------------------------------- plan object
+struct lf2mi_fft_plan_s {
+    int N, C, rh, kh, dtype;
+    int device;
+    void *tw;        // N complex twiddles (dtype)
+    double *green_s; // N doubles
+    void *work;      // backward / Poisson workspace: N * N rows of wcols complex
+    int64_t wcols;   // padded columns per workspace row
+    int64_t bytes;
+};
+
+namespace lf2mi {
+
+template <typename T, int N>
+static int launch_strided(const StridedArgs<T> &A, int S, int mode, int nbatch, hipStream_t st) {
+    constexpr int WG = Tune<T>::WG;
+    constexpr int L = Tune<T>::template strided_L<N>();
+    const size_t shm = sizeof(cplx<T>) * N * L;
+    dim3 grid((A.ncols + L - 1) / L, nbatch);
+    LF2MI_PROF(mode == 1 ? "fft_y_green" : (S < 0 ? "fft_strided_fwd" : "fft_strided_inv"), st);
+    if (mode == 1)
+        fft_strided_kernel<T, N, L, WG, -1, 1><<<grid, WG, shm, st>>>(A);
+    else if (S < 0)
+        fft_strided_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
+    else
+        fft_strided_kernel<T, N, L, WG, +1, 0><<<grid, WG, shm, st>>>(A);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+template <typename T, int N>
+static int launch_x(const XArgs<T> &A, int S, hipStream_t st) {
+    constexpr int WG = Tune<T>::WG;
+    constexpr int L = Tune<T>::template x_L<N>();
+    if (A.C > L) return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", A.C, L);
+    const int P = L / A.C;
+    const int64_t npairs = (int64_t)N * N / 2;
+    const unsigned grid = (unsigned)((npairs + P - 1) / P);
+    const size_t shm = sizeof(cplx<T>) * N * (L + 1);
+    LF2MI_PROF(S < 0 ? "fft_x_r2c" : "fft_x_c2r", st);
+    if (S < 0)
+        fft_x_r2c_kernel<T, N, L, WG><<<grid, WG, shm, st>>>(A);
+    else
+        fft_x_c2r_kernel<T, N, L, WG><<<grid, WG, shm, st>>>(A);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+#define LF2MI_DISPATCH_N(N_, CALL)                              \
+    switch (N_) {                                                \
+        case 8: { constexpr int NN = 8; return CALL; }           \
+        case 16: { constexpr int NN = 16; return CALL; }         \
+        case 32: { constexpr int NN = 32; return CALL; }         \
+        case 64: { constexpr int NN = 64; return CALL; }         \
+        case 128: { constexpr int NN = 128; return CALL; }       \
+        case 256: { constexpr int NN = 256; return CALL; }       \
+        case 512: { constexpr int NN = 512; return CALL; }       \
+        case 1024: { constexpr int NN = 1024; return CALL; }     \
+        default: return fail(LF2MI_ERANGE, "FFT size %d not supported", N_); \
+    }
+
+template <typename T> static int strided(int N, const StridedArgs<T> &A, int S, int mode, int nb, hipStream_t st) {
+    LF2MI_DISPATCH_N(N, (launch_strided<T, NN>(A, S, mode, nb, st)));
+}
+template <typename T> static int xpass(int N, const XArgs<T> &A, int S, hipStream_t st) {
+    LF2MI_DISPATCH_N(N, (launch_x<T, NN>(A, S, st)));
+}
+
+// k-field geometry
+struct KGeo {
+    int64_t col0, s1, s2;  // complex units
+};
+static KGeo kfield_geo(const lf2mi_fft_plan_s *p) {
+    const int64_t R = p->N / 2 + 1 + 2 * p->kh, KG = p->N + 2 * p->kh;
+    KGeo g;
+    g.s1 = R * p->C;
+    g.s2 = R * KG * p->C;
+    g.col0 = (int64_t)p->kh * p->C + (int64_t)p->kh * g.s1 + (int64_t)p->kh * g.s2;
+    return g;
+}
+static KGeo work_geo(const lf2mi_fft_plan_s *p) {
+    KGeo g;
+    g.col0 = 0;
+    g.s1 = p->wcols;
+    g.s2 = p->wcols * p->N;
+    return g;
+}
+
+template <typename T>
+static int forward_x_only(lf2mi_fft_plan_s *p, const void *rfield, cplx<T> *kout, KGeo kg, T shift, hipStream_t st) {
+    XArgs<T> X{};
+    X.real_in = rfield;
+    X.kout = kout;
+    X.C = p->C;
+    X.rh = p->rh;
+    X.k_col0 = kg.col0;
+    X.k_s1 = kg.s1;
+    X.k_s2 = kg.s2;
+    X.shift = shift;
+    X.post = T(1);
+    X.tw = (const cplx<T> *)p->tw;
+    return xpass<T>(p->N, X, -1, st);
+}
+
+template <typename T>
+static int forward(lf2mi_fft_plan_s *p, const void *rfield, cplx<T> *kout, KGeo kg, T shift, hipStream_t st) {
+    const int N = p->N;
+    if (int rc = forward_x_only<T>(p, rfield, kout, kg, shift, st)) return rc;
+    StridedArgs<T> A{};
+    A.in = kout;
+    A.out = kout;
+    A.in_col0 = A.out_col0 = kg.col0;
+    A.ncols = (N / 2 + 1) * p->C;
+    A.tw = (const cplx<T> *)p->tw;
+    A.C = p->C;
+    // z pass: along slot1, batches over slot2
+    A.in_es = A.out_es = kg.s1;
+    A.in_bs = A.out_bs = kg.s2;
+    if (int rc = strided<T>(N, A, -1, 0, N, st)) return rc;
+    // y pass: along slot2, batches over slot1
+    A.in_es = A.out_es = kg.s2;
+    A.in_bs = A.out_bs = kg.s1;
+    return strided<T>(N, A, -1, 0, N, st);
+}
+
+template <typename T>
+static int backward(lf2mi_fft_plan_s *p, const cplx<T> *kin, KGeo kg, void *rfield, T post, hipStream_t st) {
+    const int N = p->N;
+    cplx<T> *W = (cplx<T> *)p->work;
+    const KGeo wg = work_geo(p);
+    StridedArgs<T> A{};
+    A.ncols = (N / 2 + 1) * p->C;
+    A.tw = (const cplx<T> *)p->tw;
+    A.C = p->C;
+    // y pass: k-field -> W along slot2
+    A.in = kin;
+    A.out = W;
+    A.in_col0 = kg.col0;
+    A.out_col0 = wg.col0;
+    A.in_es = kg.s2;
+    A.out_es = wg.s2;
+    A.in_bs = kg.s1;
+    A.out_bs = wg.s1;
+    if (int rc = strided<T>(N, A, +1, 0, N, st)) return rc;
+    // z pass in place on W along slot1
+    A.in = W;
+    A.in_col0 = wg.col0;
+    A.in_es = A.out_es = wg.s1;
+    A.in_bs = A.out_bs = wg.s2;
+    if (int rc = strided<T>(N, A, +1, 0, N, st)) return rc;
+    XArgs<T> X{};
+    X.kin = W;
+    X.real_out = (T *)rfield;
+    X.C = p->C;
+    X.rh = p->rh;
+    X.k_col0 = wg.col0;
+    X.k_s1 = wg.s1;
+    X.k_s2 = wg.s2;
+    X.post = post;
+    X.tw = (const cplx<T> *)p->tw;
+    return xpass<T>(N, X, +1, st);
+}
+
+template <typename T>
+static int poisson(lf2mi_fft_plan_s *p, const void *rho, void *phi, double shift, double scale, hipStream_t st) {
+    const int N = p->N;
+    cplx<T> *W = (cplx<T> *)p->work;
+    const KGeo wg = work_geo(p);
+    if (int rc = forward_x_only<T>(p, rho, W, wg, (T)shift, st)) return rc;
+    StridedArgs<T> A{};
+    A.in = W;
+    A.out = W;
+    A.in_col0 = A.out_col0 = wg.col0;
+    A.ncols = (N / 2 + 1) * p->C;
+    A.tw = (const cplx<T> *)p->tw;
+    A.C = p->C;
+    A.green_s = p->green_s;
+    A.green_scale = scale;
+    // z forward along slot1
+    A.in_es = A.out_es = wg.s1;
+    A.in_bs = A.out_bs = wg.s2;
+    if (int rc = strided<T>(N, A, -1, 0, N, st)) return rc;
+    // y forward + Green + y inverse along slot2 (batch = slot1 = kz)
+    A.in_es = A.out_es = wg.s2;
+    A.in_bs = A.out_bs = wg.s1;
+    if (int rc = strided<T>(N, A, -1, 1, N, st)) return rc;
+    // z inverse along slot1
+    A.in_es = A.out_es = wg.s1;
+    A.in_bs = A.out_bs = wg.s2;
+    if (int rc = strided<T>(N, A, +1, 0, N, st)) return rc;
+    XArgs<T> X{};
+    X.kin = W;
+    X.real_out = (T *)phi;
+    X.C = p->C;
+    X.rh = p->rh;
+    X.k_col0 = wg.col0;
+    X.k_s1 = wg.s1;
+    X.k_s2 = wg.s2;
+    X.post = T(1);
+    X.tw = (const cplx<T> *)p->tw;
+    return xpass<T>(N, X, +1, st);
+}
+
+// ---------------------------------------------------------------- Green kernel (drop-in)
+template <typename T>
+__global__ void green_kernel(cplx<T> *__restrict__ k, int N, int C, int kh, double inv_res2, double scale,
+                             const double *__restrict__ cosv) {
+    const int H = N / 2 + 1;
+    const int64_t R = H + 2 * kh, KG = N + 2 * kh;
+    const int64_t total = (int64_t)H * N * N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const int kx = (int)(t % H);
+        const int kz = (int)((t / H) % N);
+        const int ky = (int)(t / ((int64_t)H * N));
+        const int64_t o = ((kx + kh) + R * ((kz + kh) + KG * (ky + kh))) * C;
+        double f = 0.0;
+        if (kx | ky | kz) {
+            const double lam = 2.0 * (cosv[kx] + cosv[ky] + cosv[kz] - 3.0) * inv_res2;
+            f = scale / lam;
+        }
+        for (int c = 0; c < C; ++c) {
+            cplx<T> v = k[o + c];
+            v.x = (T)((double)v.x * f);
+            v.y = (T)((double)v.y * f);
+            k[o + c] = v;
+        }
+    }
+}
+
+}  // namespace lf2mi
+
+using namespace lf2mi;
+
+extern "C" {
+
+int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan) {
+    if (!plan) return fail(LF2MI_EINVAL, "lf2mi_fft_plan_create: null plan pointer");
+    *plan = nullptr;
+    if (!is_pow2(N) || N < 8 || N > 1024) return fail(LF2MI_ERANGE, "FFT size N=%d: a power of two in [8, 1024] is required", N);
+    if (C < 1 || rhalo < 0 || khalo < 0) return fail(LF2MI_EINVAL, "bad plan geometry C=%d rhalo=%d khalo=%d", C, rhalo, khalo);
+    if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
+    const int xl = dtype == LF2MI_F32 ? (N >= 512 ? 6 : (N >= 128 ? 12 : 24)) : (N >= 512 ? 3 : (N >= 128 ? 6 : 12));
+    if (C > xl) return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", C, xl);
+    lf2mi_fft_plan_s *p = new (std::nothrow) lf2mi_fft_plan_s();
+    if (!p) return fail(LF2MI_ENOMEM, "host allocation failed");
+    p->N = N;
+    p->C = C;
+    p->rh = rhalo;
+    p->kh = khalo;
+    p->dtype = dtype;
+    hipGetDevice(&p->device);
+    const size_t esz = dtype == LF2MI_F32 ? sizeof(float2) : sizeof(double2);
+    // twiddles and Green eigenvalues in FP64
+    double *twd = (double *)malloc(sizeof(double) * 2 * N);
+    double *gs = (double *)malloc(sizeof(double) * N);
+    float *twf = (float *)malloc(sizeof(float) * 2 * N);
+    if (!twd || !gs || !twf) {
+        free(twd); free(gs); free(twf); delete p;
+        return fail(LF2MI_ENOMEM, "host allocation failed");
+    }
+    for (int q = 0; q < N; ++q) {
+        const double a = 2.0 * M_PI * (double)q / (double)N;
+        twd[2 * q] = cos(a);
+        twd[2 * q + 1] = sin(a);
+        twf[2 * q] = (float)twd[2 * q];
+        twf[2 * q + 1] = (float)twd[2 * q + 1];
+        gs[q] = (double)N * N * (2.0 - 2.0 * cos(a));
+    }
+    // workspace rows padded to a multiple of 16 complex values (128 B for float2)
+    p->wcols = (((int64_t)(N / 2 + 1) * C + 15) / 16) * 16;
+    const int64_t wbytes = (int64_t)esz * p->wcols * N * N;
+    int rc = LF2MI_OK;
+    hipError_t e;
+    if ((e = hipMalloc(&p->tw, esz * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(twiddles)", __FILE__, __LINE__);
+    if (!rc && (e = hipMalloc((void **)&p->green_s, sizeof(double) * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(green)", __FILE__, __LINE__);
+    if (!rc && (e = hipMalloc(&p->work, wbytes)) != hipSuccess) rc = hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+    if (!rc && (e = hipMemcpy(p->tw, dtype == LF2MI_F32 ? (void *)twf : (void *)twd, esz * N, hipMemcpyHostToDevice)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy(twiddles)", __FILE__, __LINE__);
+    if (!rc && (e = hipMemcpy(p->green_s, gs, sizeof(double) * N, hipMemcpyHostToDevice)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy(green)", __FILE__, __LINE__);
+    free(twd);
+    free(gs);
+    free(twf);
+    if (rc) {
+        hipFree(p->tw);
+        hipFree(p->green_s);
+        hipFree(p->work);
+        delete p;
+        return rc;
+    }
+    p->bytes = (int64_t)esz * N + (int64_t)sizeof(double) * N + wbytes;
+    *plan = p;
+    return LF2MI_OK;
+}
+
+int lf2mi_fft_plan_destroy(lf2mi_fft_plan p) {
+    if (!p) return LF2MI_OK;
+    hipFree(p->tw);
+    hipFree(p->green_s);
+    hipFree(p->work);
+    delete p;
+    return LF2MI_OK;
+}
+
+int lf2mi_fft_plan_bytes(lf2mi_fft_plan p, int64_t *bytes) {
+    if (!p || !bytes) return fail(LF2MI_EINVAL, "lf2mi_fft_plan_bytes: null argument");
+    *bytes = p->bytes;
+    return LF2MI_OK;
+}
+
+int lf2mi_fft_execute(lf2mi_fft_plan p, int direction, void *rfield, void *kfield, void *stream) {
+    if (!p) return fail(LF2MI_EINVAL, "lf2mi_fft_execute: null plan");
+    if (direction != LF2MI_FFT_FORWARD && direction != LF2MI_FFT_BACKWARD) return LF2MI_OK;
+    if (!rfield || !kfield) return fail(LF2MI_EINVAL, "lf2mi_fft_execute: null field");
+    hipStream_t st = as_stream(stream);
+    const KGeo kg = kfield_geo(p);
+    if (p->dtype == LF2MI_F32) {
+        if (direction == LF2MI_FFT_FORWARD) return forward<float>(p, rfield, (float2 *)kfield, kg, 0.0f, st);
+        return backward<float>(p, (const float2 *)kfield, kg, rfield, 1.0f, st);
+    }
+    if (direction == LF2MI_FFT_FORWARD) return forward<double>(p, rfield, (double2 *)kfield, kg, 0.0, st);
+    return backward<double>(p, (const double2 *)kfield, kg, rfield, 1.0, st);
+}
+
+int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, double scale, int dtype, void *stream) {
+    if (!kfield || N < 1 || C < 1 || khalo < 0 || !(res > 0)) return fail(LF2MI_EINVAL, "lf2mi_poisson_green: bad argument");
+    double *cosv = nullptr;
+    double *host = (double *)malloc(sizeof(double) * N);
+    if (!host) return fail(LF2MI_ENOMEM, "host allocation failed");
+    for (int q = 0; q < N; ++q) host[q] = cos(2.0 * M_PI * q / N);
+    hipStream_t st = as_stream(stream);
+    hipError_t e = hipMallocAsync((void **)&cosv, sizeof(double) * N, st);
+    if (e != hipSuccess) { free(host); return hip_fail(e, "hipMallocAsync", __FILE__, __LINE__); }
+    e = hipMemcpyAsync(cosv, host, sizeof(double) * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    free(host);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    const int64_t total = (int64_t)(N / 2 + 1) * N * N;
+    const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+    LF2MI_PROF("poisson_green", st);
+    if (dtype == LF2MI_F32)
+        green_kernel<float><<<g, 256, 0, st>>>((float2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, cosv);
+    else if (dtype == LF2MI_F64)
+        green_kernel<double><<<g, 256, 0, st>>>((double2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, cosv);
+    else {
+        hipFreeAsync(cosv, st);
+        return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
+    }
+    LF2MI_LAUNCHED();
+    LF2MI_HIP(hipFreeAsync(cosv, st));
+    return LF2MI_OK;
+}
+
+int lf2mi_poisson_solve(lf2mi_fft_plan p, const void *rho, void *phi, double shift, double scale, void *stream) {
+    if (!p || !rho || !phi) return fail(LF2MI_EINVAL, "lf2mi_poisson_solve: null argument");
+    if (p->C != 1) return fail(LF2MI_EINVAL, "lf2mi_poisson_solve needs a one-component plan");
+    if (p->dtype == LF2MI_F32) return poisson<float>(p, rho, phi, shift, scale, as_stream(stream));
+    return poisson<double>(p, rho, phi, shift, scale, as_stream(stream));
+}
+
+}  // extern "C"

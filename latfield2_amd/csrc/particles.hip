@@ -1,0 +1,1381 @@
+// Particle container, CIC deposits and the fused gather + kick + drift of the lf2mi
+// library (Particles<part_simple,...> and particles/projections.hpp on one gfx950 device).
+//
+// F32 representation: per particle an int32 cell index x + N(y + N z) and an FP32 offset
+// in cell units in [0,1) per axis (position = (cell + offset)/N, formed in FP64 when the
+// particle moves), FP32 velocity, optional FP32 mass and int64 ID, in SoA arrays.
+// Particles are bucketed by bricks of B^3 cells (B = 8 when N allows); a bucket is the
+// unit of locality that the reference gets from its per-cell std::forward_list
+// (particles/LATfield2_Particles.hpp:97-103,333).  Deposits and gathers work on "items":
+// chunks of at most ITEM_P particles of one bucket, each staged through an LDS window
+// of the brick's nodes.  After a drift the container is re-bucketed with a counting sort
+// (histogram -> scan -> scatter into the second buffer).
+//
+// F64 representation (the plain path): absolute FP64 positions and velocities in upload
+// order; deposits are per-particle FP64 atomics into the field and the gather reads phi
+// directly.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+namespace lf2mi {
+
+static constexpr int ITEM_P = 1024;     // particles per item
+static constexpr int DEP_WG = 256;      // deposit / gather workgroup size
+static constexpr int MAXB = 8;          // largest brick edge
+
+// ---------------------------------------------------------------- RNG (counter based)
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// uniform double in [0,1) from (seed, index, stream)
+__host__ __device__ inline double urand(uint64_t seed, uint64_t idx, uint32_t stream) {
+    const uint64_t h = mix64(seed ^ mix64(idx * 16ull + stream));
+    return (double)(h >> 11) * 0x1.0p-53;
+}
+__device__ inline double grand(uint64_t seed, uint64_t idx, uint32_t stream) {
+    // Box-Muller on two uniforms of the same counter
+    double u1 = urand(seed, idx, stream);
+    const double u2 = urand(seed, idx, stream + 8);
+    if (u1 < 0x1.0p-60) u1 = 0x1.0p-60;
+    return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+// ---------------------------------------------------------------- exact FP64 helpers
+// Rounded add / multiply that the compiler may not fuse into an FMA: the drift must
+// round exactly like the reference's `pos += dtau * vel` (one product, one sum).
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double sub_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// wave-aggregated atomic increment: lanes with the same key share one atomic; returns
+// each lane's slot (base + rank among the lanes with its key)
+__device__ __forceinline__ int agg_inc(int *counters, int key, bool active) {
+    unsigned long long todo = __ballot(active);
+    int slot = 0;
+    const int lane = lane_id();
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long grp = __ballot(active && key == lkey);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lkey], __popcll(grp));
+        base = __shfl(base, leader);
+        if (active && key == lkey) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+        todo &= ~grp;
+    }
+    return slot;
+}
+
+// ---------------------------------------------------------------- geometry helpers
+struct Geo {
+    int N, B, NB;
+    int64_t nbk;
+};
+__device__ __forceinline__ int bucket_of(int cell, const Geo &g) {
+    const int x = cell % g.N, y = (cell / g.N) % g.N, z = cell / (g.N * g.N);
+    return (x / g.B) + g.NB * ((y / g.B) + g.NB * (z / g.B));
+}
+__device__ __forceinline__ int wrapi(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
+
+// ---------------------------------------------------------------- F32 SoA buffer
+struct Soa32 {
+    int32_t *cell;
+    float *off[3];
+    float *vel[3];
+    float *mass;   // may be null
+    int64_t *id;   // may be null
+};
+
+// ---------------------------------------------------------------- binning (FP64 -> cell, offset)
+// Particles::getPartCoord: (int)floor(pos*N) % N (particles/LATfield2_Particles.hpp:464-467).
+// The offset is pos*N - floor(pos*N) rounded to FP32 and kept below 1.
+__device__ __forceinline__ bool bin_one(const double p[3], int N, int &cell, float off[3]) {
+    int c[3];
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+        const double t = p[d] * (double)N;
+        const double fl = floor(t);
+        c[d] = (int)fl % N;
+        if (c[d] < 0) ok = false;
+        float o = (float)(t - fl);
+        if (!(o < 1.0f)) o = 0x1.fffffep-1f;
+        off[d] = o;
+    }
+    cell = ok ? c[0] + N * (c[1] + N * c[2]) : -1;
+    return ok;
+}
+
+// staged FP64 input: AoS pos/vel (host layout), optional mass/id
+__global__ void bin_upload_kernel(int64_t np, const double *__restrict__ pos, const double *__restrict__ vel,
+                                  const double *__restrict__ mass, const int64_t *__restrict__ id, Soa32 dst,
+                                  Geo g, int *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < np;
+    int cell = -1;
+    if (in) {
+        const double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        float off[3];
+        bin_one(p, g.N, cell, off);
+        dst.cell[i] = cell;
+        for (int d = 0; d < 3; ++d) {
+            dst.off[d][i] = off[d];
+            dst.vel[d][i] = (float)vel[3 * i + d];
+        }
+        if (dst.mass) dst.mass[i] = (float)mass[i];
+        if (dst.id) dst.id[i] = id ? id[i] : i;
+    }
+    const bool act = in && cell >= 0;
+    agg_inc(counts, act ? bucket_of(cell, g) : 0, act);
+}
+
+// ---------------------------------------------------------------- synthetic ICs (F32 and F64)
+struct IcArgs {
+    int kind;  // 0 lattice, 1 uniform, 2 blobs
+    int64_t np;
+    int ng;
+    double sigma_pos;  // box units
+    double sigma_v;
+    uint64_t seed;
+    int nblobs;
+    int64_t nclustered;
+    const double *centres;   // 3 * nblobs
+    const int64_t *bstart;   // nblobs + 1 prefix of blob occupancies
+};
+
+__device__ __forceinline__ void ic_particle(const IcArgs &A, int64_t i, double p[3], double v[3]) {
+    if (A.kind == 0) {
+        const int64_t ng = A.ng;
+        const int64_t ix = i % ng, iy = (i / ng) % ng, iz = i / (ng * ng);
+        const double base[3] = {(double)ix / ng, (double)iy / ng, (double)iz / ng};
+        for (int d = 0; d < 3; ++d) p[d] = base[d] + A.sigma_pos * grand(A.seed, i, d);
+    } else if (A.kind == 1 || i >= A.nclustered) {
+        for (int d = 0; d < 3; ++d) p[d] = urand(A.seed, i, 4 + d);
+    } else {
+        // blob of particle i: largest b with bstart[b] <= i
+        int lo = 0, hi = A.nblobs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (A.bstart[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        for (int d = 0; d < 3; ++d) p[d] = A.centres[3 * lo + d] + A.sigma_pos * grand(A.seed, i, d);
+    }
+    for (int d = 0; d < 3; ++d) {
+        p[d] -= floor(p[d]);  // periodic wrap into [0, 1)
+        if (p[d] >= 1.0) p[d] = 0.0;
+        v[d] = A.sigma_v * grand(A.seed, i, 12 + d);
+    }
+}
+
+__global__ void ic_f32_kernel(IcArgs A, Soa32 dst, Geo g, int *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < A.np;
+    int cell = -1;
+    if (in) {
+        double p[3], v[3];
+        ic_particle(A, i, p, v);
+        float off[3];
+        bin_one(p, g.N, cell, off);
+        dst.cell[i] = cell;
+        for (int d = 0; d < 3; ++d) {
+            dst.off[d][i] = off[d];
+            dst.vel[d][i] = (float)v[d];
+        }
+        if (dst.mass) dst.mass[i] = 1.0f;
+        if (dst.id) dst.id[i] = i;
+    }
+    const bool act = in && cell >= 0;
+    agg_inc(counts, act ? bucket_of(cell, g) : 0, act);
+}
+
+struct Soa64 {
+    int32_t *cell;
+    double *pos[3];
+    double *vel[3];
+    double *mass;
+    int64_t *id;
+};
+
+__global__ void ic_f64_kernel(IcArgs A, Soa64 dst, int N) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    double p[3], v[3];
+    ic_particle(A, i, p, v);
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        dst.pos[d][i] = p[d];
+        dst.vel[d][i] = v[d];
+        c[d] = (int)floor(p[d] * (double)N) % N;
+    }
+    dst.cell[i] = c[0] + N * (c[1] + N * c[2]);
+    if (dst.mass) dst.mass[i] = 1.0;
+    if (dst.id) dst.id[i] = i;
+}
+
+__global__ void upload_f64_kernel(int64_t np, const double *__restrict__ pos, const double *__restrict__ vel,
+                                  const double *__restrict__ mass, const int64_t *__restrict__ id, Soa64 dst, int N) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        const double p = pos[3 * i + d];
+        dst.pos[d][i] = p;
+        dst.vel[d][i] = vel[3 * i + d];
+        c[d] = (int)floor(p * (double)N) % N;
+    }
+    dst.cell[i] = c[0] + N * (c[1] + N * c[2]);
+    if (dst.mass) dst.mass[i] = mass[i];
+    if (dst.id) dst.id[i] = id ? id[i] : i;
+}
+
+// ---------------------------------------------------------------- scans
+// exclusive scans of count[] (-> start[]) and of ceil(count / ITEM_P) (-> istart[]),
+// n <= 4096 * 1024, each with the total at index n.
+static constexpr int SCAN_T = 256, SCAN_PER = 4, SCAN_CHUNK = SCAN_T * SCAN_PER;
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total) {
+    // Hillis-Steele over SCAN_T threads
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_T; o <<= 1) {
+        const int add = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    total = sh[SCAN_T - 1];
+    const int incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void scan_local_kernel(const int *__restrict__ count, int64_t n, int *__restrict__ start,
+                                  int *__restrict__ istart, int *__restrict__ bsum) {
+    __shared__ int sh[SCAN_T];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER;
+    int c[SCAN_PER], it[SCAN_PER];
+    int sc = 0, si = 0;
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int64_t i = base + k;
+        c[k] = i < n ? count[i] : 0;
+        it[k] = (c[k] + ITEM_P - 1) / ITEM_P;
+        sc += c[k];
+        si += it[k];
+    }
+    int tc, ti;
+    int oc = block_excl_scan(sc, sh, tc);
+    int oi = block_excl_scan(si, sh, ti);
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int64_t i = base + k;
+        if (i < n) {
+            start[i] = oc;
+            istart[i] = oi;
+        }
+        oc += c[k];
+        oi += it[k];
+    }
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = tc;
+        bsum[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+__global__ void scan_sums_kernel(int *__restrict__ bsum, int nblocks, int *__restrict__ start,
+                                 int *__restrict__ istart, int64_t n) {
+    // one block of 1024 threads, up to 4 block sums per thread
+    __shared__ int sc[1024], si[1024];
+    const int t = threadIdx.x;
+    int vc[4], vi[4], tc = 0, ti = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int b = t * 4 + k;
+        vc[k] = b < nblocks ? bsum[2 * b] : 0;
+        vi[k] = b < nblocks ? bsum[2 * b + 1] : 0;
+        tc += vc[k];
+        ti += vi[k];
+    }
+    sc[t] = tc;
+    si[t] = ti;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int ac = t >= o ? sc[t - o] : 0;
+        const int ai = t >= o ? si[t - o] : 0;
+        __syncthreads();
+        sc[t] += ac;
+        si[t] += ai;
+        __syncthreads();
+    }
+    int oc = sc[t] - tc, oi = si[t] - ti;
+    for (int k = 0; k < 4; ++k) {
+        const int b = t * 4 + k;
+        if (b < nblocks) {
+            bsum[2 * b] = oc;
+            bsum[2 * b + 1] = oi;
+        }
+        oc += vc[k];
+        oi += vi[k];
+    }
+    if (t == 1023) {
+        start[n] = sc[1023];
+        istart[n] = si[1023];
+    }
+}
+
+__global__ void scan_add_kernel(int *__restrict__ start, int *__restrict__ istart, const int *__restrict__ bsum,
+                                int64_t n) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK;
+    const int ac = bsum[2 * blockIdx.x], ai = bsum[2 * blockIdx.x + 1];
+    for (int k = threadIdx.x; k < SCAN_CHUNK; k += SCAN_T) {
+        const int64_t i = base + k;
+        if (i < n) {
+            start[i] += ac;
+            istart[i] += ai;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- scatter (counting sort)
+__global__ void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g, const int *__restrict__ start,
+                               int *__restrict__ cursor) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < np;
+    const int cell = in ? src.cell[i] : -1;
+    const bool act = in && cell >= 0;
+    const int b = act ? bucket_of(cell, g) : 0;
+    const int slot = agg_inc(cursor, b, act);
+    if (!act) return;
+    const int64_t j = (int64_t)start[b] + slot;
+    dst.cell[j] = cell;
+    for (int d = 0; d < 3; ++d) {
+        dst.off[d][j] = src.off[d][i];
+        dst.vel[d][j] = src.vel[d][i];
+    }
+    if (src.mass) dst.mass[j] = src.mass[i];
+    if (src.id) dst.id[j] = src.id[i];
+}
+
+// ---------------------------------------------------------------- items
+__device__ __forceinline__ int find_bucket(const int *__restrict__ istart, int64_t nbk, int item) {
+    int64_t lo = 0, hi = nbk;  // istart[lo] <= item < istart[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (istart[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return (int)lo;
+}
+
+struct ItemRange {
+    int b;        // bucket
+    int p0, p1;   // particle range
+    int ox, oy, oz;  // brick origin (cells)
+};
+__device__ __forceinline__ bool item_range(int item, const int *__restrict__ start, const int *__restrict__ istart,
+                                           const Geo &g, ItemRange &r) {
+    if (item >= istart[g.nbk]) return false;
+    const int b = find_bucket(istart, g.nbk, item);
+    const int k = item - istart[b];
+    r.b = b;
+    r.p0 = start[b] + k * ITEM_P;
+    r.p1 = min(r.p0 + ITEM_P, start[b + 1]);
+    r.ox = (b % g.NB) * g.B;
+    r.oy = ((b / g.NB) % g.NB) * g.B;
+    r.oz = (int)(b / ((int64_t)g.NB * g.NB)) * g.B;
+    return true;
+}
+
+// ---------------------------------------------------------------- deposit (F32, LDS window)
+// Window nodes [o-1, o+B] per axis (W = B+2) for each of the C components.
+// KIND 0 scalar (C=1), 1 vector (C=3), 2 vecvec (C=6).  TGT 0 unfolded, 1 periodic.
+template <int KIND> struct KindC;
+template <> struct KindC<0> { static constexpr int C = 1; };
+template <> struct KindC<1> { static constexpr int C = 3; };
+template <> struct KindC<2> { static constexpr int C = 6; };
+
+struct AxisW {
+    int lo;      // node offset of the lower node relative to the cell (-1 or 0)
+    float w0, w1;
+};
+__device__ __forceinline__ AxisW vtx_w(float u) {  // vertex grid
+    AxisW a;
+    a.lo = 0;
+    a.w0 = 1.0f - u;
+    a.w1 = u;
+    return a;
+}
+__device__ __forceinline__ AxisW link_w(float u) {  // link grid, nodes at n + 1/2
+    AxisW a;
+    const float t = u - 0.5f;
+    if (t < 0.0f) {
+        a.lo = -1;
+        a.w0 = -t;
+        a.w1 = 1.0f - a.w0;
+    } else {
+        a.lo = 0;
+        a.w1 = t;
+        a.w0 = 1.0f - a.w1;
+    }
+    return a;
+}
+
+template <typename ACC>
+__device__ __forceinline__ void lds_add(ACC *win, int idx, float v, double fxscale);
+template <>
+__device__ __forceinline__ void lds_add<float>(float *win, int idx, float v, double) {
+    atomicAdd(&win[idx], v);
+}
+template <>
+__device__ __forceinline__ void lds_add<unsigned long long>(unsigned long long *win, int idx, float v, double fxscale) {
+    const long long q = llrint((double)v * fxscale);
+    atomicAdd(&win[idx], (unsigned long long)q);
+}
+
+template <typename ACC>
+__device__ __forceinline__ void add8(ACC *win, int W, int comp, int lx, int ly, int lz, const AxisW &ax,
+                                     const AxisW &ay, const AxisW &az, float q, double fxscale) {
+    const int W3 = W * W * W;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const float w = (i ? ax.w1 : ax.w0) * (j ? ay.w1 : ay.w0) * (k ? az.w1 : az.w0);
+                const int idx = comp * W3 + (lx + ax.lo + i + 1) + W * ((ly + ay.lo + j + 1) + W * (lz + az.lo + k + 1));
+                lds_add<ACC>(win, idx, q * w, fxscale);
+            }
+}
+
+struct DepArgs {
+    Soa32 s;
+    const int *start;
+    const int *istart;
+    Geo g;
+    float mass_nv;        // global mass * N^3 (mass / res^3)
+    float *field;         // native accumulation target
+    unsigned long long *fx; // fixed-point target (KIND 0 only)
+    double fxscale;       // 2^32 / (mass_nv_ref) for the fixed path
+    int h;
+    int tgt;
+};
+
+template <int KIND, typename ACC>
+__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
+    constexpr int C = KindC<KIND>::C;
+    const int W = A.g.B + 2;
+    const int W3 = W * W * W;
+    __shared__ ACC win[C * (MAXB + 2) * (MAXB + 2) * (MAXB + 2)];
+    ItemRange r;
+    if (!item_range(blockIdx.x, A.start, A.istart, A.g, r)) return;
+    for (int i = threadIdx.x; i < C * W3; i += DEP_WG) win[i] = ACC(0);
+    __syncthreads();
+    const int N = A.g.N;
+    for (int p = r.p0 + threadIdx.x; p < r.p1; p += DEP_WG) {
+        const int cell = A.s.cell[p];
+        const int lx = cell % N - r.ox, ly = (cell / N) % N - r.oy, lz = cell / (N * N) - r.oz;
+        const float u[3] = {A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]};
+        const float m = A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv;
+        const AxisW v0 = vtx_w(u[0]), v1 = vtx_w(u[1]), v2 = vtx_w(u[2]);
+        if constexpr (KIND == 0) {
+            add8<ACC>(win, W, 0, lx, ly, lz, v0, v1, v2, m, A.fxscale);
+        } else {
+            const float vel[3] = {A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p]};
+            const AxisW l0 = link_w(u[0]), l1 = link_w(u[1]), l2 = link_w(u[2]);
+            if constexpr (KIND == 1) {
+                add8<ACC>(win, W, 0, lx, ly, lz, l0, v1, v2, m * vel[0], A.fxscale);
+                add8<ACC>(win, W, 1, lx, ly, lz, v0, l1, v2, m * vel[1], A.fxscale);
+                add8<ACC>(win, W, 2, lx, ly, lz, v0, v1, l2, m * vel[2], A.fxscale);
+            } else {
+                add8<ACC>(win, W, 0, lx, ly, lz, v0, v1, v2, m * vel[0] * vel[0], A.fxscale);
+                add8<ACC>(win, W, 3, lx, ly, lz, v0, v1, v2, m * vel[1] * vel[1], A.fxscale);
+                add8<ACC>(win, W, 5, lx, ly, lz, v0, v1, v2, m * vel[2] * vel[2], A.fxscale);
+                add8<ACC>(win, W, 1, lx, ly, lz, l0, l1, v2, m * vel[0] * vel[1], A.fxscale);
+                add8<ACC>(win, W, 2, lx, ly, lz, l0, v1, l2, m * vel[0] * vel[2], A.fxscale);
+                add8<ACC>(win, W, 4, lx, ly, lz, v0, l1, l2, m * vel[1] * vel[2], A.fxscale);
+            }
+        }
+    }
+    __syncthreads();
+    // flush the window: node (o - 1 + w) per axis
+    for (int i = threadIdx.x; i < C * W3; i += DEP_WG) {
+        const ACC v = win[i];
+        if (v == ACC(0)) continue;
+        const int c = i / W3;
+        const int s = i % W3;
+        int gx = r.ox - 1 + s % W, gy = r.oy - 1 + (s / W) % W, gz = r.oz - 1 + s / (W * W);
+        if (A.tgt == LF2MI_DEP_PERIODIC) {
+            gx = wrapi(gx, N);
+            gy = wrapi(gy, N);
+            gz = wrapi(gz, N);
+        }
+        const int64_t idx = site_index(N, A.h, gx, gy, gz) * C + c;
+        if constexpr (sizeof(ACC) == 4) atomicAdd(&A.field[idx], (float)v);
+        else atomicAdd(&A.fx[idx], (unsigned long long)v);
+    }
+}
+
+__global__ void fixed_to_field_kernel(const unsigned long long *__restrict__ fx, float *__restrict__ f, int64_t n,
+                                      double scale) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const long long q = (long long)fx[i];
+        if (q) f[i] += (float)((double)q * scale);
+    }
+}
+
+// ---------------------------------------------------------------- gather + kick + drift (F32)
+struct GkdArgs {
+    Soa32 s;
+    const int *start;
+    const int *istart;
+    Geo g;
+    const float *phi;
+    int h;
+    float dt_kick;
+    double dt_drift;
+    int *next_counts;       // histogram of the new buckets
+    unsigned int *vmax2;    // bits of max |v|^2 (non-negative float)
+};
+
+__global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
+    constexpr int PW = MAXB + 3;  // phi window [o-1, o+B+1]
+    constexpr int FW = MAXB + 1;  // force nodes [o, o+B]
+    __shared__ float phw[PW * PW * PW];
+    __shared__ float frc[3][FW * FW * FW];
+    __shared__ unsigned int red;
+    ItemRange r;
+    if (!item_range(blockIdx.x, A.start, A.istart, A.g, r)) return;
+    const int N = A.g.N, B = A.g.B;
+    const int pw = B + 3, fw = B + 1;
+    if (threadIdx.x == 0) red = 0u;
+    for (int i = threadIdx.x; i < pw * pw * pw; i += DEP_WG) {
+        const int gx = wrapi(r.ox - 1 + i % pw, N);
+        const int gy = wrapi(r.oy - 1 + (i / pw) % pw, N);
+        const int gz = wrapi(r.oz - 1 + i / (pw * pw), N);
+        phw[i] = A.phi[site_index(N, A.h, gx, gy, gz)];
+    }
+    __syncthreads();
+    const float hn = 0.5f * (float)N;
+    for (int i = threadIdx.x; i < fw * fw * fw; i += DEP_WG) {
+        const int x = i % fw + 1, y = (i / fw) % fw + 1, z = i / (fw * fw) + 1;
+        const int c = x + pw * (y + pw * z);
+        frc[0][i] = -(phw[c + 1] - phw[c - 1]) * hn;
+        frc[1][i] = -(phw[c + pw] - phw[c - pw]) * hn;
+        frc[2][i] = -(phw[c + pw * pw] - phw[c - pw * pw]) * hn;
+    }
+    __syncthreads();
+    float vmax = 0.0f;
+    for (int base = r.p0; base < r.p1; base += DEP_WG) {
+        const int p = base + threadIdx.x;
+        const bool act = p < r.p1;
+        int newcell = 0;
+        if (act) {
+            const int cell = A.s.cell[p];
+            const int cx = cell % N, cy = (cell / N) % N, cz = cell / (N * N);
+            const int lx = cx - r.ox, ly = cy - r.oy, lz = cz - r.oz;
+            const float u[3] = {A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]};
+            float a[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const float w = (i ? u[0] : 1.0f - u[0]) * (j ? u[1] : 1.0f - u[1]) * (k ? u[2] : 1.0f - u[2]);
+                        const int n = (lx + i) + fw * ((ly + j) + fw * (lz + k));
+                        a[0] += w * frc[0][n];
+                        a[1] += w * frc[1][n];
+                        a[2] += w * frc[2][n];
+                    }
+            float v[3];
+            int c3[3] = {cx, cy, cz};
+            float o[3];
+            for (int d = 0; d < 3; ++d) {
+                v[d] = A.s.vel[d][p] + A.dt_kick * a[d];
+                // drift in FP64 cell units: X = (c + u) + fl(dt v) N, then the sequential
+                // periodic wrap of Particles.hpp:2028-2032 and re-binning
+                double X = add_rn((double)c3[d], (double)u[d]);
+                X = add_rn(X, mul_rn(A.dt_drift, (double)v[d]) * (double)N);
+                if (X < 0.0) X = add_rn(X, (double)N);
+                if (X >= (double)N) X = sub_rn(X, (double)N);
+                // beyond the reference's range (|dt v| >= box) wrap fully so the cell stays valid
+                if (X < 0.0 || X >= (double)N) X -= (double)N * floor(X / (double)N);
+                const double fl = floor(X);
+                int cn = (int)fl;
+                if (cn >= N) cn = N - 1;
+                if (cn < 0) cn = 0;
+                float on = (float)(X - fl);
+                if (!(on < 1.0f)) on = 0x1.fffffep-1f;
+                c3[d] = cn;
+                o[d] = on;
+            }
+            newcell = c3[0] + N * (c3[1] + N * c3[2]);
+            A.s.cell[p] = newcell;
+            for (int d = 0; d < 3; ++d) {
+                A.s.off[d][p] = o[d];
+                A.s.vel[d][p] = v[d];
+            }
+            vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        }
+        agg_inc(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act);
+    }
+    if (A.vmax2) {
+        atomicMax(&red, __float_as_uint(vmax));
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(A.vmax2, red);
+    }
+}
+
+// ---------------------------------------------------------------- F64 plain path
+struct Dep64Args {
+    Soa64 s;
+    int64_t np;
+    int N;
+    double mass_nv;
+    double *field;
+    int h, tgt;
+};
+
+__device__ __forceinline__ void add8_64(double *f, int C, int comp, int N, int h, int tgt, int cx, int cy, int cz,
+                                        int lox, int loy, int loz, const double wx[2], const double wy[2],
+                                        const double wz[2], double q) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                int x = cx + lox + i, y = cy + loy + j, z = cz + loz + k;
+                if (tgt == LF2MI_DEP_PERIODIC) {
+                    x = wrapi(x, N);
+                    y = wrapi(y, N);
+                    z = wrapi(z, N);
+                }
+                atomicAdd(&f[site_index(N, h, x, y, z) * C + comp], q * wx[i] * wy[j] * wz[k]);
+            }
+}
+
+template <int KIND>
+__global__ void deposit_f64_kernel(Dep64Args A) {
+    constexpr int C = KindC<KIND>::C;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.np) return;
+    const int N = A.N;
+    int c[3];
+    double vw[3][2], lw[3][2];
+    int llo[3];
+    for (int d = 0; d < 3; ++d) {
+        const double t = A.s.pos[d][p] * (double)N;
+        c[d] = (int)floor(t) % N;
+        const double u = t - floor(t);
+        vw[d][0] = 1.0 - u;
+        vw[d][1] = u;
+        const double s = u - 0.5;
+        if (s < 0) {
+            llo[d] = -1;
+            lw[d][0] = -s;
+            lw[d][1] = 1.0 - lw[d][0];
+        } else {
+            llo[d] = 0;
+            lw[d][1] = s;
+            lw[d][0] = 1.0 - lw[d][1];
+        }
+    }
+    const double m = A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv;
+    if constexpr (KIND == 0) {
+        add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m);
+    } else {
+        const double v[3] = {A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p]};
+        if constexpr (KIND == 1) {
+            add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], llo[0], 0, 0, lw[0], vw[1], vw[2], m * v[0]);
+            add8_64(A.field, C, 1, N, A.h, A.tgt, c[0], c[1], c[2], 0, llo[1], 0, vw[0], lw[1], vw[2], m * v[1]);
+            add8_64(A.field, C, 2, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, llo[2], vw[0], vw[1], lw[2], m * v[2]);
+        } else {
+            add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[0] * v[0]);
+            add8_64(A.field, C, 3, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[1] * v[1]);
+            add8_64(A.field, C, 5, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[2] * v[2]);
+            add8_64(A.field, C, 1, N, A.h, A.tgt, c[0], c[1], c[2], llo[0], llo[1], 0, lw[0], lw[1], vw[2], m * v[0] * v[1]);
+            add8_64(A.field, C, 2, N, A.h, A.tgt, c[0], c[1], c[2], llo[0], 0, llo[2], lw[0], vw[1], lw[2], m * v[0] * v[2]);
+            add8_64(A.field, C, 4, N, A.h, A.tgt, c[0], c[1], c[2], 0, llo[1], llo[2], vw[0], lw[1], lw[2], m * v[1] * v[2]);
+        }
+    }
+}
+
+struct Gkd64Args {
+    Soa64 s;
+    int64_t np;
+    int N, h;
+    const double *phi;
+    double dt_kick, dt_drift;
+    unsigned long long *vmax2;  // bits of max |v|^2 (non-negative double)
+};
+
+__global__ void gkd_f64_kernel(Gkd64Args A) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.np) return;
+    const int N = A.N;
+    int c[3];
+    double u[3];
+    for (int d = 0; d < 3; ++d) {
+        const double t = A.s.pos[d][p] * (double)N;
+        c[d] = (int)floor(t) % N;
+        u[d] = t - (double)(int)floor(t);
+    }
+    const double hn = 0.5 * N;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const double w = (i ? u[0] : 1.0 - u[0]) * (j ? u[1] : 1.0 - u[1]) * (k ? u[2] : 1.0 - u[2]);
+                const int x = c[0] + i, y = c[1] + j, z = c[2] + k;
+                auto P = [&](int xx, int yy, int zz) {
+                    return A.phi[site_index(N, A.h, wrapi(xx, N), wrapi(yy, N), wrapi(zz, N))];
+                };
+                a[0] += w * (-(P(x + 1, y, z) - P(x - 1, y, z)) * hn);
+                a[1] += w * (-(P(x, y + 1, z) - P(x, y - 1, z)) * hn);
+                a[2] += w * (-(P(x, y, z + 1) - P(x, y, z - 1)) * hn);
+            }
+    double v2 = 0.0;
+    int cn[3];
+    for (int d = 0; d < 3; ++d) {
+        const double v = add_rn(A.s.vel[d][p], mul_rn(A.dt_kick, a[d]));
+        A.s.vel[d][p] = v;
+        v2 += v * v;
+        double x = add_rn(A.s.pos[d][p], mul_rn(A.dt_drift, v));
+        if (x < 0.0) x = add_rn(x, 1.0);
+        if (x >= 1.0) x = sub_rn(x, 1.0);
+        // beyond the reference's range (|dt v| >= box) wrap fully so the cell stays valid
+        if (x < 0.0 || x >= 1.0) x -= floor(x);
+        A.s.pos[d][p] = x;
+        cn[d] = (int)floor(x * (double)N) % N;
+    }
+    A.s.cell[p] = cn[0] + N * (cn[1] + N * cn[2]);
+    if (A.vmax2) atomicMax(A.vmax2, (unsigned long long)__double_as_longlong(v2));
+}
+
+// ---------------------------------------------------------------- download / counts
+__global__ void download_f32_kernel(int64_t np, Soa32 s, int N, double *__restrict__ pos, double *__restrict__ vel,
+                                    double *__restrict__ mass) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int cell = s.cell[i];
+    const int c[3] = {cell % N, (cell / N) % N, cell / (N * N)};
+    for (int d = 0; d < 3; ++d) {
+        if (pos) pos[3 * i + d] = add_rn((double)c[d], (double)s.off[d][i]) / (double)N;
+        if (vel) vel[3 * i + d] = (double)s.vel[d][i];
+    }
+    if (mass) mass[i] = s.mass ? (double)s.mass[i] : 1.0;
+}
+
+__global__ void download_f64_kernel(int64_t np, Soa64 s, double *__restrict__ pos, double *__restrict__ vel) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    for (int d = 0; d < 3; ++d) {
+        if (pos) pos[3 * i + d] = s.pos[d][i];
+        if (vel) vel[3 * i + d] = s.vel[d][i];
+    }
+}
+
+__global__ void cell_count_kernel(int64_t np, const int32_t *__restrict__ cell, int32_t *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np && cell[i] >= 0) atomicAdd(&counts[cell[i]], 1);
+}
+
+__global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace lf2mi
+
+// ---------------------------------------------------------------- container
+struct lf2mi_parts_s {
+    int N, dtype, flags, device;
+    lf2mi::Geo g;
+    int64_t cap, np;
+    double mass_global;
+    double total_mass;   // sum of masses (PM mean density with box volume 1)
+    lf2mi::Soa32 b32[2];
+    int cur;
+    lf2mi::Soa64 b64;
+    int *bstart, *bcount, *istart, *bsum;
+    unsigned long long *fx;
+    int64_t fx_n;
+    unsigned int *vmax_f32;
+    unsigned long long *vmax_f64;
+    std::vector<void *> allocs;
+    int64_t bytes;
+};
+
+namespace lf2mi {
+
+static int dalloc(lf2mi_parts_s *p, void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) return LF2MI_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(particles)", __FILE__, __LINE__);
+    p->allocs.push_back(*ptr);
+    p->bytes += (int64_t)bytes;
+    return LF2MI_OK;
+}
+
+static unsigned grid1(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+static int fill_int(int *ptr, int64_t n, int v, hipStream_t st) {
+    if (n <= 0) return LF2MI_OK;
+    LF2MI_PROF("fill", st);
+    fill_int_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(ptr, n, v);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+// exclusive scans of bcount -> bstart, items -> istart
+static int scan_buckets(lf2mi_parts_s *p, hipStream_t st) {
+    const int64_t n = p->g.nbk;
+    const int nblocks = (int)((n + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    if (nblocks > 4096) return fail(LF2MI_ERANGE, "too many buckets (%lld)", (long long)n);
+    LF2MI_PROF("rebucket_scan", st);
+    scan_local_kernel<<<nblocks, SCAN_T, 0, st>>>(p->bcount, n, p->bstart, p->istart, p->bsum);
+    LF2MI_LAUNCHED();
+    scan_sums_kernel<<<1, 1024, 0, st>>>(p->bsum, nblocks, p->bstart, p->istart, n);
+    LF2MI_LAUNCHED();
+    scan_add_kernel<<<nblocks, SCAN_T, 0, st>>>(p->bstart, p->istart, p->bsum, n);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+// counting sort of buffer `src` (counts already in bcount) into buffer 1 - src
+static int sort_into_other(lf2mi_parts_s *p, int src, int64_t nsrc, hipStream_t st) {
+    if (int rc = scan_buckets(p, st)) return rc;
+    if (int rc = fill_int(p->bcount, p->g.nbk, 0, st)) return rc;
+    LF2MI_PROF("rebucket_scatter", st);
+    scatter_kernel<<<grid1(nsrc), 256, 0, st>>>(nsrc, p->b32[src], p->b32[1 - src], p->g, p->bstart, p->bcount);
+    LF2MI_LAUNCHED();
+    p->cur = 1 - src;
+    return LF2MI_OK;
+}
+
+static int read_total(lf2mi_parts_s *p, hipStream_t st, int64_t *out) {
+    int tot = 0;
+    LF2MI_HIP(hipMemcpyAsync(&tot, p->bstart + p->g.nbk, sizeof(int), hipMemcpyDeviceToHost, st));
+    LF2MI_HIP(hipStreamSynchronize(st));
+    *out = tot;
+    return LF2MI_OK;
+}
+
+static int64_t item_grid(const lf2mi_parts_s *p) { return p->g.nbk + p->np / ITEM_P + 1; }
+
+static int ic_common(lf2mi_parts_s *p, IcArgs &A, hipStream_t st) {
+    if (A.np > p->cap) return fail(LF2MI_ERANGE, "%lld particles exceed the capacity %lld", (long long)A.np, (long long)p->cap);
+    if (p->dtype == LF2MI_F32) {
+        if (int rc = fill_int(p->bcount, p->g.nbk, 0, st)) return rc;
+        const int src = 1 - p->cur;
+        ic_f32_kernel<<<grid1(A.np), 256, 0, st>>>(A, p->b32[src], p->g, p->bcount);
+        LF2MI_LAUNCHED();
+        if (int rc = sort_into_other(p, src, A.np, st)) return rc;
+        if (int rc = read_total(p, st, &p->np)) return rc;
+    } else {
+        ic_f64_kernel<<<grid1(A.np), 256, 0, st>>>(A, p->b64, p->N);
+        LF2MI_LAUNCHED();
+        LF2MI_HIP(hipStreamSynchronize(st));
+        p->np = A.np;
+    }
+    p->total_mass = p->mass_global * (double)p->np;  // synthetic particles carry unit mass ratio
+    return LF2MI_OK;
+}
+
+}  // namespace lf2mi
+
+using namespace lf2mi;
+
+extern "C" {
+
+int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_parts *out) {
+    if (!out) return fail(LF2MI_EINVAL, "lf2mi_parts_create: null output");
+    *out = nullptr;
+    if (N < 2 || N > 1290) return fail(LF2MI_ERANGE, "lattice size N=%d outside [2, 1290]", N);
+    if (capacity < 0 || capacity > 2000000000LL) return fail(LF2MI_ERANGE, "capacity %lld outside [0, 2e9]", (long long)capacity);
+    if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
+    lf2mi_parts_s *p = new (std::nothrow) lf2mi_parts_s();
+    if (!p) return fail(LF2MI_ENOMEM, "host allocation failed");
+    p->N = N;
+    p->dtype = dtype;
+    p->flags = flags;
+    hipGetDevice(&p->device);
+    p->cap = capacity;
+    p->np = 0;
+    p->mass_global = 1.0;
+    p->cur = 0;
+    int B = 1;
+    for (int b = MAXB; b >= 1; --b)
+        if (N % b == 0) { B = b; break; }
+    p->g.N = N;
+    p->g.B = B;
+    p->g.NB = N / B;
+    p->g.nbk = (int64_t)p->g.NB * p->g.NB * p->g.NB;
+    const size_t cap = (size_t)std::max<int64_t>(capacity, 1);
+    int rc = LF2MI_OK;
+    auto A = [&](void **ptr, size_t bytes) {
+        if (!rc) rc = dalloc(p, ptr, bytes);
+    };
+    if (dtype == LF2MI_F32) {
+        for (int k = 0; k < 2; ++k) {
+            Soa32 &s = p->b32[k];
+            A((void **)&s.cell, cap * sizeof(int32_t));
+            for (int d = 0; d < 3; ++d) {
+                A((void **)&s.off[d], cap * sizeof(float));
+                A((void **)&s.vel[d], cap * sizeof(float));
+            }
+            s.mass = nullptr;
+            s.id = nullptr;
+            if (flags & LF2MI_PARTS_MASS) A((void **)&s.mass, cap * sizeof(float));
+            if (flags & LF2MI_PARTS_ID) A((void **)&s.id, cap * sizeof(int64_t));
+        }
+        const size_t nb = (size_t)p->g.nbk + 1;
+        A((void **)&p->bstart, nb * sizeof(int));
+        A((void **)&p->istart, nb * sizeof(int));
+        A((void **)&p->bcount, nb * sizeof(int));
+        A((void **)&p->bsum, 2 * 4096 * sizeof(int));
+        A((void **)&p->vmax_f32, sizeof(unsigned int));
+    } else {
+        Soa64 &s = p->b64;
+        A((void **)&s.cell, cap * sizeof(int32_t));
+        for (int d = 0; d < 3; ++d) {
+            A((void **)&s.pos[d], cap * sizeof(double));
+            A((void **)&s.vel[d], cap * sizeof(double));
+        }
+        s.mass = nullptr;
+        s.id = nullptr;
+        if (flags & LF2MI_PARTS_MASS) A((void **)&s.mass, cap * sizeof(double));
+        if (flags & LF2MI_PARTS_ID) A((void **)&s.id, cap * sizeof(int64_t));
+        A((void **)&p->vmax_f64, sizeof(unsigned long long));
+    }
+    if (rc) {
+        for (void *q : p->allocs) hipFree(q);
+        delete p;
+        return rc;
+    }
+    if (dtype == LF2MI_F32) {
+        hipMemset(p->bstart, 0, ((size_t)p->g.nbk + 1) * sizeof(int));
+        hipMemset(p->istart, 0, ((size_t)p->g.nbk + 1) * sizeof(int));
+    }
+    *out = p;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_destroy(lf2mi_parts p) {
+    if (!p) return LF2MI_OK;
+    for (void *q : p->allocs) hipFree(q);
+    if (p->fx) hipFree(p->fx);
+    delete p;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_set_mass(lf2mi_parts p, double m) {
+    if (!p) return fail(LF2MI_EINVAL, "null container");
+    if (p->total_mass != 0.0 && !(p->flags & LF2MI_PARTS_MASS)) p->total_mass = m * (double)p->np;
+    p->mass_global = m;
+    if (!(p->flags & LF2MI_PARTS_MASS)) p->total_mass = m * (double)p->np;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_count(lf2mi_parts p, int64_t *np) {
+    if (!p || !np) return fail(LF2MI_EINVAL, "null argument");
+    *np = p->np;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_bytes(lf2mi_parts p, int64_t *bytes) {
+    if (!p || !bytes) return fail(LF2MI_EINVAL, "null argument");
+    *bytes = p->bytes;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const double *vel, const double *mass,
+                       const int64_t *id, int64_t *kept, void *stream) {
+    if (!p || np < 0 || (np > 0 && (!pos || !vel))) return fail(LF2MI_EINVAL, "lf2mi_parts_upload: bad argument");
+    if (np > p->cap) return fail(LF2MI_ERANGE, "%lld particles exceed the capacity %lld", (long long)np, (long long)p->cap);
+    if ((p->flags & LF2MI_PARTS_MASS) && np > 0 && !mass) return fail(LF2MI_EINVAL, "per-particle masses required");
+    hipStream_t st = as_stream(stream);
+    // drop what the reference's setCoord rejects (negative cell coordinates) on the host,
+    // so the device arrays hold only valid particles
+    std::vector<int64_t> keep;
+    keep.reserve((size_t)np);
+    for (int64_t i = 0; i < np; ++i) {
+        bool ok = true;
+        for (int d = 0; d < 3; ++d)
+            if ((int)floor(pos[3 * i + d] * (double)p->N) % p->N < 0) ok = false;
+        if (ok) keep.push_back(i);
+    }
+    const int64_t nk = (int64_t)keep.size();
+    std::vector<double> hp((size_t)nk * 3), hv((size_t)nk * 3), hm(mass ? (size_t)nk : 0);
+    std::vector<int64_t> hid((size_t)nk);
+    double msum = 0.0;
+    for (int64_t k = 0; k < nk; ++k) {
+        const int64_t i = keep[(size_t)k];
+        for (int d = 0; d < 3; ++d) {
+            hp[3 * k + d] = pos[3 * i + d];
+            hv[3 * k + d] = vel[3 * i + d];
+        }
+        if (mass) {
+            hm[(size_t)k] = mass[i];
+            msum += mass[i];
+        }
+        hid[(size_t)k] = id ? id[i] : i;
+    }
+    double *dpos = nullptr, *dvel = nullptr, *dmass = nullptr;
+    int64_t *did = nullptr;
+    const size_t n3 = (size_t)std::max<int64_t>(nk, 1) * 3;
+    LF2MI_HIP(hipMalloc((void **)&dpos, n3 * sizeof(double)));
+    LF2MI_HIP(hipMalloc((void **)&dvel, n3 * sizeof(double)));
+    LF2MI_HIP(hipMalloc((void **)&did, (n3 / 3) * sizeof(int64_t)));
+    if (mass) LF2MI_HIP(hipMalloc((void **)&dmass, (n3 / 3) * sizeof(double)));
+    int rc = LF2MI_OK;
+    hipError_t e = hipSuccess;
+    if (nk > 0) {
+        e = hipMemcpyAsync(dpos, hp.data(), nk * 3 * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dvel, hv.data(), nk * 3 * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(did, hid.data(), nk * sizeof(int64_t), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && mass) e = hipMemcpyAsync(dmass, hm.data(), nk * sizeof(double), hipMemcpyHostToDevice, st);
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(upload)", __FILE__, __LINE__);
+    if (!rc) {
+        if (p->dtype == LF2MI_F32) {
+            rc = fill_int(p->bcount, p->g.nbk, 0, st);
+            const int src = 1 - p->cur;
+            if (!rc && nk > 0) {
+                bin_upload_kernel<<<grid1(nk), 256, 0, st>>>(nk, dpos, dvel, dmass, did, p->b32[src], p->g, p->bcount);
+                if ((e = hipGetLastError()) != hipSuccess) rc = hip_fail(e, "bin_upload_kernel", __FILE__, __LINE__);
+            }
+            if (!rc) rc = sort_into_other(p, src, nk, st);
+            if (!rc) rc = read_total(p, st, &p->np);
+        } else {
+            if (nk > 0) {
+                upload_f64_kernel<<<grid1(nk), 256, 0, st>>>(nk, dpos, dvel, dmass, did, p->b64, p->N);
+                if ((e = hipGetLastError()) != hipSuccess) rc = hip_fail(e, "upload_f64_kernel", __FILE__, __LINE__);
+            }
+            if (!rc && (e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_fail(e, "sync", __FILE__, __LINE__);
+            p->np = nk;
+        }
+    }
+    hipStreamSynchronize(st);
+    hipFree(dpos);
+    hipFree(dvel);
+    hipFree(did);
+    if (dmass) hipFree(dmass);
+    if (rc) return rc;
+    p->total_mass = mass ? msum : p->mass_global * (double)p->np;
+    if (kept) *kept = p->np;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, int64_t *id, int32_t *cell,
+                         void *stream) {
+    if (!p) return fail(LF2MI_EINVAL, "null container");
+    hipStream_t st = as_stream(stream);
+    const int64_t np = p->np;
+    if (np == 0) return LF2MI_OK;
+    double *dpos = nullptr, *dvel = nullptr, *dmass = nullptr;
+    LF2MI_HIP(hipMalloc((void **)&dpos, np * 3 * sizeof(double)));
+    LF2MI_HIP(hipMalloc((void **)&dvel, np * 3 * sizeof(double)));
+    LF2MI_HIP(hipMalloc((void **)&dmass, np * sizeof(double)));
+    int rc = LF2MI_OK;
+    if (p->dtype == LF2MI_F32) {
+        const Soa32 &s = p->b32[p->cur];
+        download_f32_kernel<<<grid1(np), 256, 0, st>>>(np, s, p->N, dpos, dvel, dmass);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && pos) e = hipMemcpyAsync(pos, dpos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, dvel, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && mass) e = hipMemcpyAsync(mass, dmass, np * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && id) {
+            if (s.id) e = hipMemcpyAsync(id, s.id, np * sizeof(int64_t), hipMemcpyDeviceToHost, st);
+            else rc = fail(LF2MI_EINVAL, "container keeps no IDs");
+        }
+        if (e == hipSuccess && cell) e = hipMemcpyAsync(cell, s.cell, np * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) rc = hip_fail(e, "download", __FILE__, __LINE__);
+    } else {
+        const Soa64 &s = p->b64;
+        download_f64_kernel<<<grid1(np), 256, 0, st>>>(np, s, dpos, dvel);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && pos) e = hipMemcpyAsync(pos, dpos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, dvel, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && mass) {
+            if (s.mass) e = hipMemcpyAsync(mass, s.mass, np * sizeof(double), hipMemcpyDeviceToHost, st);
+            else for (int64_t i = 0; i < np; ++i) mass[i] = p->mass_global;
+        }
+        if (e == hipSuccess && id) {
+            if (s.id) e = hipMemcpyAsync(id, s.id, np * sizeof(int64_t), hipMemcpyDeviceToHost, st);
+            else rc = fail(LF2MI_EINVAL, "container keeps no IDs");
+        }
+        if (e == hipSuccess && cell) e = hipMemcpyAsync(cell, s.cell, np * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) rc = hip_fail(e, "download", __FILE__, __LINE__);
+    }
+    hipStreamSynchronize(st);
+    hipFree(dpos);
+    hipFree(dvel);
+    hipFree(dmass);
+    return rc;
+}
+
+int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream) {
+    if (!p || !counts) return fail(LF2MI_EINVAL, "null argument");
+    hipStream_t st = as_stream(stream);
+    const int64_t nc = (int64_t)p->N * p->N * p->N;
+    if (int rc = fill_int(counts, nc, 0, st)) return rc;
+    const int32_t *cell = p->dtype == LF2MI_F32 ? p->b32[p->cur].cell : p->b64.cell;
+    if (p->np > 0) {
+        cell_count_kernel<<<grid1(p->np), 256, 0, st>>>(p->np, cell, counts);
+        LF2MI_LAUNCHED();
+    }
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_ic_lattice(lf2mi_parts p, int ng, double sigma_cells, double sigma_v, uint64_t seed, void *stream) {
+    if (!p || ng < 1) return fail(LF2MI_EINVAL, "bad argument");
+    IcArgs A{};
+    A.kind = 0;
+    A.np = (int64_t)ng * ng * ng;
+    A.ng = ng;
+    A.sigma_pos = sigma_cells / p->N;
+    A.sigma_v = sigma_v;
+    A.seed = seed;
+    return ic_common(p, A, as_stream(stream));
+}
+
+int lf2mi_parts_ic_uniform(lf2mi_parts p, int64_t np, double sigma_v, uint64_t seed, void *stream) {
+    if (!p || np < 0) return fail(LF2MI_EINVAL, "bad argument");
+    IcArgs A{};
+    A.kind = 1;
+    A.np = np;
+    A.sigma_v = sigma_v;
+    A.seed = seed;
+    return ic_common(p, A, as_stream(stream));
+}
+
+int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha, double sigma_cells, double frac_uniform,
+                         double sigma_v, uint64_t seed, void *stream) {
+    if (!p || np < 0 || nblobs < 1 || !(alpha > 0) || frac_uniform < 0 || frac_uniform > 1)
+        return fail(LF2MI_EINVAL, "bad argument");
+    const int64_t ncl = np - (int64_t)llround(frac_uniform * (double)np);
+    // Pareto(alpha) occupancies w = u^(-1/alpha), counts by largest remainder
+    std::vector<double> w((size_t)nblobs), cen((size_t)nblobs * 3);
+    double ws = 0.0;
+    for (int b = 0; b < nblobs; ++b) {
+        double u = urand(seed ^ 0xB10Bull, (uint64_t)b, 0);
+        if (u < 1e-300) u = 1e-300;
+        w[(size_t)b] = pow(u, -1.0 / alpha);
+        ws += w[(size_t)b];
+        for (int d = 0; d < 3; ++d) cen[(size_t)b * 3 + d] = urand(seed ^ 0xCE47ull, (uint64_t)b, (uint32_t)d);
+    }
+    std::vector<int64_t> cnt((size_t)nblobs);
+    std::vector<std::pair<double, int>> rem((size_t)nblobs);
+    int64_t assigned = 0;
+    for (int b = 0; b < nblobs; ++b) {
+        const double exact = w[(size_t)b] / ws * (double)ncl;
+        cnt[(size_t)b] = (int64_t)floor(exact);
+        assigned += cnt[(size_t)b];
+        rem[(size_t)b] = {exact - floor(exact), b};
+    }
+    std::sort(rem.begin(), rem.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+        return a.first > b.first || (a.first == b.first && a.second < b.second);
+    });
+    for (int64_t k = 0; k < ncl - assigned; ++k) cnt[(size_t)rem[(size_t)(k % nblobs)].second]++;
+    std::vector<int64_t> bst((size_t)nblobs + 1, 0);
+    for (int b = 0; b < nblobs; ++b) bst[(size_t)b + 1] = bst[(size_t)b] + cnt[(size_t)b];
+    hipStream_t st = as_stream(stream);
+    double *dcen = nullptr;
+    int64_t *dbst = nullptr;
+    LF2MI_HIP(hipMalloc((void **)&dcen, cen.size() * sizeof(double)));
+    LF2MI_HIP(hipMalloc((void **)&dbst, bst.size() * sizeof(int64_t)));
+    LF2MI_HIP(hipMemcpy(dcen, cen.data(), cen.size() * sizeof(double), hipMemcpyHostToDevice));
+    LF2MI_HIP(hipMemcpy(dbst, bst.data(), bst.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    IcArgs A{};
+    A.kind = 2;
+    A.np = np;
+    A.sigma_pos = sigma_cells / p->N;
+    A.sigma_v = sigma_v;
+    A.seed = seed;
+    A.nblobs = nblobs;
+    A.nclustered = ncl;
+    A.centres = dcen;
+    A.bstart = dbst;
+    const int rc = ic_common(p, A, st);
+    hipStreamSynchronize(st);
+    hipFree(dcen);
+    hipFree(dbst);
+    return rc;
+}
+
+int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, void *stream) {
+    if (!p || !field) return fail(LF2MI_EINVAL, "lf2mi_deposit: null argument");
+    if (kind < LF2MI_PROJ_SCALAR || kind > LF2MI_PROJ_VECVEC) return fail(LF2MI_EINVAL, "bad projection kind %d", kind);
+    if (halo < 1) return fail(LF2MI_EINVAL, "the field has to have at least a halo of 1");
+    if (target != LF2MI_DEP_UNFOLDED && target != LF2MI_DEP_PERIODIC) return fail(LF2MI_EINVAL, "bad target %d", target);
+    if (acc != LF2MI_ACC_NATIVE && acc != LF2MI_ACC_FIXED64) return fail(LF2MI_EINVAL, "bad accumulation mode %d", acc);
+    hipStream_t st = as_stream(stream);
+    const int N = p->N;
+    const double nv = (double)N * N * N;
+    if (p->np == 0) return LF2MI_OK;
+    if (p->dtype == LF2MI_F64) {
+        if (acc != LF2MI_ACC_NATIVE) return fail(LF2MI_EUNSUPPORTED, "fixed-point deposit is an F32 option");
+        Dep64Args A{};
+        A.s = p->b64;
+        A.np = p->np;
+        A.N = N;
+        A.mass_nv = p->mass_global * nv;
+        A.field = (double *)field;
+        A.h = halo;
+        A.tgt = target;
+        if (p->b64.mass) A.mass_nv = nv;
+        const unsigned g = grid1(p->np);
+        LF2MI_PROF(kind == 0 ? "deposit_f64_scalar" : (kind == 1 ? "deposit_f64_vector" : "deposit_f64_vecvec"), st);
+        if (kind == 0) deposit_f64_kernel<0><<<g, 256, 0, st>>>(A);
+        else if (kind == 1) deposit_f64_kernel<1><<<g, 256, 0, st>>>(A);
+        else deposit_f64_kernel<2><<<g, 256, 0, st>>>(A);
+        LF2MI_LAUNCHED();
+        return LF2MI_OK;
+    }
+    DepArgs A{};
+    A.s = p->b32[p->cur];
+    A.start = p->bstart;
+    A.istart = p->istart;
+    A.g = p->g;
+    A.mass_nv = (float)((p->flags & LF2MI_PARTS_MASS) ? nv : p->mass_global * nv);
+    A.field = (float *)field;
+    A.h = halo;
+    A.tgt = target;
+    const unsigned g = (unsigned)item_grid(p);
+    if (acc == LF2MI_ACC_FIXED64) {
+        if (kind != LF2MI_PROJ_SCALAR) return fail(LF2MI_EUNSUPPORTED, "fixed-point deposit supports the scalar projection");
+        const int64_t G = N + 2 * halo;
+        const int64_t n = G * G * G;
+        if (p->fx_n < n) {
+            if (p->fx) hipFree(p->fx);
+            p->fx = nullptr;
+            p->fx_n = 0;
+            LF2MI_HIP(hipMalloc((void **)&p->fx, n * sizeof(unsigned long long)));
+            p->fx_n = n;
+        }
+        LF2MI_HIP(hipMemsetAsync(p->fx, 0, n * sizeof(unsigned long long), st));
+        // accumulate dimensionless weights (times the per-particle mass ratio) in 2^-32 units
+        const double mref = A.mass_nv;
+        A.fx = p->fx;
+        A.fxscale = 4294967296.0 / mref;
+        {
+            LF2MI_PROF("deposit_f32_fixed64", st);
+            deposit_f32_kernel<0, unsigned long long><<<g, DEP_WG, 0, st>>>(A);
+            LF2MI_LAUNCHED();
+        }
+        {
+            LF2MI_PROF("fixed_to_field", st);
+            fixed_to_field_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(
+                p->fx, (float *)field, n, mref / 4294967296.0);
+            LF2MI_LAUNCHED();
+        }
+        return LF2MI_OK;
+    }
+    LF2MI_PROF(kind == 0 ? "deposit_f32_scalar" : (kind == 1 ? "deposit_f32_vector" : "deposit_f32_vecvec"), st);
+    if (kind == 0) deposit_f32_kernel<0, float><<<g, DEP_WG, 0, st>>>(A);
+    else if (kind == 1) deposit_f32_kernel<1, float><<<g, DEP_WG, 0, st>>>(A);
+    else deposit_f32_kernel<2, float><<<g, DEP_WG, 0, st>>>(A);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick, double dt_drift, double *vmax2,
+                            void *stream) {
+    if (!p || !phi || halo < 0) return fail(LF2MI_EINVAL, "lf2mi_gather_kick_drift: bad argument");
+    hipStream_t st = as_stream(stream);
+    if (p->np == 0) {
+        if (vmax2) *vmax2 = 0.0;
+        return LF2MI_OK;
+    }
+    if (p->dtype == LF2MI_F64) {
+        Gkd64Args A{};
+        A.s = p->b64;
+        A.np = p->np;
+        A.N = p->N;
+        A.h = halo;
+        A.phi = (const double *)phi;
+        A.dt_kick = dt_kick;
+        A.dt_drift = dt_drift;
+        A.vmax2 = vmax2 ? p->vmax_f64 : nullptr;
+        if (vmax2) LF2MI_HIP(hipMemsetAsync(p->vmax_f64, 0, sizeof(unsigned long long), st));
+        LF2MI_PROF("gkd_f64", st);
+        gkd_f64_kernel<<<grid1(p->np), 256, 0, st>>>(A);
+        LF2MI_LAUNCHED();
+        if (vmax2) {
+            unsigned long long bits = 0;
+            LF2MI_HIP(hipMemcpyAsync(&bits, p->vmax_f64, sizeof(bits), hipMemcpyDeviceToHost, st));
+            LF2MI_HIP(hipStreamSynchronize(st));
+            double v;
+            memcpy(&v, &bits, sizeof(v));
+            *vmax2 = v;
+        }
+        return LF2MI_OK;
+    }
+    if (int rc = fill_int(p->bcount, p->g.nbk, 0, st)) return rc;
+    GkdArgs A{};
+    A.s = p->b32[p->cur];
+    A.start = p->bstart;
+    A.istart = p->istart;
+    A.g = p->g;
+    A.phi = (const float *)phi;
+    A.h = halo;
+    A.dt_kick = (float)dt_kick;
+    A.dt_drift = dt_drift;
+    A.next_counts = p->bcount;
+    A.vmax2 = vmax2 ? p->vmax_f32 : nullptr;
+    if (vmax2) LF2MI_HIP(hipMemsetAsync(p->vmax_f32, 0, sizeof(unsigned int), st));
+    {
+    LF2MI_PROF("gkd_f32", st);
+    gkd_f32_kernel<<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
+    LF2MI_LAUNCHED();
+    }
+    if (int rc = sort_into_other(p, p->cur, p->np, st)) return rc;
+    if (vmax2) {
+        unsigned int bits = 0;
+        LF2MI_HIP(hipMemcpyAsync(&bits, p->vmax_f32, sizeof(bits), hipMemcpyDeviceToHost, st));
+        LF2MI_HIP(hipStreamSynchronize(st));
+        float v;
+        memcpy(&v, &bits, sizeof(v));
+        *vmax2 = v;
+    }
+    return LF2MI_OK;
+}
+
+int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int halo, double dt_kick, double dt_drift,
+                  int acc, void *stream) {
+    if (!p || !plan || !rho || !phi) return fail(LF2MI_EINVAL, "lf2mi_pm_step: null argument");
+    const int N = p->N;
+    const int64_t G = N + 2 * halo;
+    if (int rc = lf2mi_zero(rho, G * G * G, p->dtype, stream)) return rc;
+    if (int rc = lf2mi_deposit(p, LF2MI_PROJ_SCALAR, rho, halo, LF2MI_DEP_PERIODIC, acc, stream)) return rc;
+    // box volume 1: the mean density is the total mass
+    const double scale = 1.0 / ((double)N * N * N);
+    if (int rc = lf2mi_poisson_solve(plan, rho, phi, p->total_mass, scale, stream)) return rc;
+    return lf2mi_gather_kick_drift(p, phi, halo, dt_kick, dt_drift, nullptr, stream);
+}
+
+}  // extern "C"

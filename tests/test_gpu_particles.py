@@ -1,0 +1,192 @@
+"""GPU parity: binning, CIC deposits (+ folds), fixed-point determinism, gather + kick + drift.
+
+Oracle: oracle/ (FP64).  Both sides consume identical FP64 positions: the FP32
+container represents a particle as (cell, FP32 offset); its download returns the
+exact FP64 position (cell + offset)/N that the oracle is then given.  Bars:
+  * cell keys and per-cell counts: bit-exact (integer work);
+  * drift on identical velocities: cells bit-exact, positions within one FP32
+    offset rounding (2^-24 cell);
+  * FP64 plain path deposits / gather: <= 1e-12 relative;
+  * FP32 deposits rel-L2 <= 1e-6, gathered accelerations rel-L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import latfield2_amd as lf  # noqa: E402
+import oracle as orc  # noqa: E402
+from gpu_util import announce, real_np, rel_l2  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DT = {"f32": lf.F32, "f64": lf.F64}
+
+
+def _random_particles(n, seed, clustered=False):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3))
+    if clustered:
+        pos[: n // 2] = (0.3 + 0.02 * rng.standard_normal((n // 2, 3))) % 1.0
+    vel = rng.standard_normal((n, 3))
+    mass = rng.random(n) + 0.5
+    return pos, vel, mass
+
+
+def _upload(N, pos, vel, prec, mass=None, cap=None):
+    parts = lf.Particles(N, cap or len(pos), DT[prec], ids=True, per_particle_mass=mass is not None)
+    kept = parts.addParticles(pos, vel, mass, ids=np.arange(len(pos)))
+    return parts, kept
+
+
+def _by_id(d):
+    order = np.argsort(d["id"])
+    return {k: v[order] for k, v in d.items()}
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_binning_bit_exact_with_edge_cases(prec):
+    N = 32
+    announce(f"binning {prec} N={N}")
+    pos, vel, _ = _random_particles(20000, 1)
+    pos[0] = [0.0, 0.0, 0.0]
+    pos[1] = [1.0 - 2 ** -53, 0.5, 31 / 32]
+    pos[2] = [1.0, 0.25, 0.0]       # pos == box: the reference's key is 0
+    pos[3] = [-1e-9, 0.1, 0.1]      # negative coordinate: dropped by addParticle_global
+    pos[4] = [5 / 32, 7 / 32 - 2 ** -60, 0.5]
+    parts, kept = _upload(N, pos, vel, prec)
+    assert kept == len(pos) - 1
+    d = _by_id(parts.download())
+    keys = orc.bin_keys(pos, N)
+    valid = keys >= 0
+    assert np.array_equal(d["id"], np.nonzero(valid)[0])
+    assert np.array_equal(d["cell"], keys[valid])
+    cnt = parts.cell_counts().cpu().numpy()
+    assert np.array_equal(cnt, orc.cell_counts(pos, N))
+    if prec == "f32":
+        # the represented position is within one FP32 offset rounding of the input
+        # (pos == 1.0 is represented as 0.0, its periodic image)
+        p = pos[valid]
+        diff = np.abs(d["pos"] - p)
+        diff = np.minimum(diff, 1.0 - diff)
+        assert np.max(diff) * N <= 2.0 ** -24
+    else:
+        assert np.array_equal(d["pos"], pos[valid])
+
+
+def _oracle_deposits(pos, vel, mass, N, h, fold):
+    rho = orc.real_field(N, h, 1)
+    B = orc.real_field(N, h, 3)
+    T = orc.real_field(N, h, 6)
+    orc.scalar_project(pos, rho, N, h, mass=mass)
+    orc.vector_project(pos, vel, B, N, h, mass=mass)
+    orc.vecvec_project(pos, vel, T, N, h, mass=mass)
+    if fold:
+        orc.scalar_comm(rho, N, h)
+        orc.vector_comm(B, N, h)
+        orc.vecvec_comm(T, N, h)
+    return [rho, B, T]
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+@pytest.mark.parametrize("N,h,n,clustered,masses", [(16, 1, 5000, False, False), (32, 2, 40000, True, True),
+                                                     (24, 1, 3000, True, False)])
+@pytest.mark.parametrize("mode", ["unfolded", "unfolded+comm", "periodic"])
+def test_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mode):
+    announce(f"deposits {prec} N={N} h={h} n={n} clustered={clustered} masses={masses} mode={mode}")
+    pos, vel, mass = _random_particles(n, N + n, clustered)
+    parts, _ = _upload(N, pos, vel, prec, mass if masses else None)
+    d = parts.download()  # the positions/velocities the container represents
+    lat = lf.Lattice(3, N, h)
+    fields = [lf.Field(lat, C, DT[prec]) for C in (1, 3, 6)]
+    kinds = [lf.PROJ_SCALAR, lf.PROJ_VECTOR, lf.PROJ_VECVEC]
+    comms = [lf.scalarProjectionCIC_comm, lf.vectorProjectionCIC_comm, lf.VecVecProjectionCIC_comm]
+    for f, kd, cm in zip(fields, kinds, comms):
+        lf.projection_init(f)
+        parts.deposit(kd, f, lf.DEP_PERIODIC if mode == "periodic" else lf.DEP_UNFOLDED)
+        if mode == "unfolded+comm":
+            cm(f)
+    torch.cuda.synchronize()
+    refs = _oracle_deposits(d["pos"], d["vel"], d["mass"] if masses else None, N, h, fold=(mode != "unfolded"))
+    G = N + 2 * h
+    for f, ref in zip(fields, refs):
+        got = real_np(f)
+        if mode == "periodic":
+            got = got[h:G - h, h:G - h, h:G - h]
+            ref = ref[h:G - h, h:G - h, h:G - h]
+        assert rel_l2(got, ref) <= tol, (mode, f.C, rel_l2(got, ref))
+
+
+def test_fixed_point_deposit_is_reproducible_and_accurate():
+    N = 32
+    announce(f"fixed-point deposit N={N}")
+    pos, vel, _ = _random_particles(60000, 3, clustered=True)
+    parts, _ = _upload(N, pos, vel, "f32")
+    lat = lf.Lattice(3, N, 1)
+    a, b, c = (lf.Field(lat, 1, lf.F32) for _ in range(3))
+    for f in (a, b):
+        lf.projection_init(f)
+        parts.deposit(lf.PROJ_SCALAR, f, lf.DEP_PERIODIC, lf.ACC_FIXED64)
+    lf.projection_init(c)
+    parts.deposit(lf.PROJ_SCALAR, c, lf.DEP_PERIODIC, lf.ACC_NATIVE)
+    torch.cuda.synchronize()
+    assert torch.equal(a.t, b.t)
+    d = parts.download()
+    ref = orc.real_field(N, 1, 1)
+    orc.scalar_project(d["pos"], ref, N, 1)
+    orc.scalar_comm(ref, N, 1)
+    assert rel_l2(real_np(a)[1:N + 1, 1:N + 1, 1:N + 1], ref[1:N + 1, 1:N + 1, 1:N + 1]) <= 1e-6
+    assert rel_l2(real_np(c)[1:N + 1, 1:N + 1, 1:N + 1], ref[1:N + 1, 1:N + 1, 1:N + 1]) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_drift_cells_bit_exact_on_identical_velocities(prec):
+    """dt_kick = 0: both sides drift the same FP64 position with the same velocity"""
+    N = 64
+    announce(f"drift bit-exact {prec} N={N}")
+    rng = np.random.default_rng(17)
+    pos = rng.random((50000, 3))
+    pos[:100] = np.floor(pos[:100] * N) / N  # on cell faces
+    vel = 0.05 * rng.standard_normal((50000, 3))
+    vel[:50] = 0.0
+    parts, _ = _upload(N, pos, vel, prec)
+    d0 = _by_id(parts.download())
+    phi = lf.Field(lf.Lattice(3, N, 1), 1, DT[prec])
+    parts.gather_kick_drift(phi, dt_kick=0.0, dt_drift=0.01)
+    torch.cuda.synchronize()
+    d1 = _by_id(parts.download())
+    p = np.ascontiguousarray(d0["pos"])
+    orc.drift(p, d0["vel"], 0.01)
+    keys = orc.bin_keys(p, N)
+    assert np.array_equal(d1["cell"], keys)
+    diff = np.abs(d1["pos"] - p)
+    diff = np.minimum(diff, 1.0 - diff)
+    bound = 0.0 if prec == "f64" else 2.0 ** -24 / N
+    assert np.max(diff) <= bound
+    assert np.array_equal(parts.cell_counts().cpu().numpy(), orc.cell_counts(p, N))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_gather_kick_matches_oracle(prec, tol):
+    N = 32
+    announce(f"gather+kick {prec} N={N}")
+    pos, vel, _ = _random_particles(30000, 21, clustered=True)
+    vel *= 0.01
+    parts, _ = _upload(N, pos, vel, prec)
+    d0 = _by_id(parts.download())
+    lat = lf.Lattice(3, N, 1)
+    phi = lf.Field(lat, 1, DT[prec])
+    g = torch.Generator(device="cpu").manual_seed(9)
+    phi.t.copy_(torch.randn(phi.t.shape, generator=g, dtype=torch.float64).to(phi.t.dtype))
+    phi_np = np.ascontiguousarray(real_np(phi))
+    dt = 0.003
+    vmax2 = parts.gather_kick_drift(phi, dt_kick=dt, dt_drift=0.0, want_vmax=True)
+    torch.cuda.synchronize()
+    d1 = _by_id(parts.download())
+    orc.ghost_fill(phi_np, N, 1)  # the kernel wraps in the interior; ghosts are its periodic image
+    F = orc.force_field(phi_np, N, 1)
+    v = np.ascontiguousarray(d0["vel"].copy())
+    acc, vm = orc.gather_kick(d0["pos"], v, F, N, 1, dt_kick=dt)
+    assert rel_l2(d1["vel"] - d0["vel"], v - d0["vel"]) <= tol
+    assert abs(vmax2 - vm) <= 1e-6 * vm
+    # dt_drift = 0: positions and cells unchanged
+    assert np.array_equal(d1["cell"], d0["cell"])

@@ -1,0 +1,102 @@
+"""GPU parity of the whole PM step against the FP64 oracle (orc_pm_step).
+
+The step: zero rho, periodic CIC deposit, mean-subtracted Poisson solve with
+-1/K^2 and 1/N^3, CIC gather of -grad phi, kick, drift, re-bucket.  Inputs are
+generated on the device (counter-based RNG), downloaded, and given to the oracle
+as the exact FP64 positions the container represents.  Bars (north_star):
+  * FP64 plain path: phi and velocities <= 1e-10 relative (summation order only);
+  * FP32 path: phi, forces and velocity updates rel-L2 <= 1e-5 against FP64;
+  * cell indices bit-exact except for particles whose FP64 position sits within
+    1e-6 cells of a cell face (the FP32 kick moves them by ~1e-9 cells); those are
+    counted and must all be boundary-ambiguous.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import latfield2_amd as lf  # noqa: E402
+import oracle as orc  # noqa: E402
+from gpu_util import announce, real_np, rel_l2  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DT = {"f32": lf.F32, "f64": lf.F64}
+
+
+def _by_id(d):
+    order = np.argsort(d["id"])
+    return {k: v[order] for k, v in d.items()}
+
+
+def _run_step(N, prec, ic, dt_kick, dt_drift, seed, mass):
+    np_ = ic["np"] if "np" in ic else ic["ng"] ** 3
+    parts = lf.Particles(N, np_, DT[prec], ids=True, mass=mass)
+    if ic["kind"] == "uniform":
+        parts.ic_uniform(np_, ic.get("sigma_v", 0.0), seed)
+    elif ic["kind"] == "lattice":
+        parts.ic_lattice(ic["ng"], ic["sigma_cells"], ic["sigma_v"], seed)
+    else:
+        parts.ic_blobs(np_, ic["nblobs"], 1.5, 2.0, 0.2, ic.get("sigma_v", 0.0), seed)
+    d0 = _by_id(parts.download())
+    lat = lf.Lattice(3, N, 1)
+    rho = lf.Field(lat, 1, DT[prec])
+    phi = lf.Field(lat, 1, DT[prec])
+    plan = lf.PlanFFT(rho, None)
+    lf.pm_step(parts, plan, rho, phi, dt_kick, dt_drift)
+    torch.cuda.synchronize()
+    d1 = _by_id(parts.download())
+    return d0, d1, real_np(rho), real_np(phi)
+
+
+def _check_step(N, prec, d0, d1, rho_g, phi_g, dt_kick, dt_drift, mass, tol):
+    pos = np.ascontiguousarray(d0["pos"].copy())
+    vel = np.ascontiguousarray(d0["vel"].copy())
+    out = orc.pm_step(pos, vel, N, dt_kick, dt_drift, mass_global=mass)
+    s = slice(1, N + 1)
+    rho_o = out["rho"][s, s, s, 0]            # mean subtracted
+    rg = rho_g[s, s, s, 0]
+    assert rel_l2(rg - rg.mean(), rho_o) <= tol
+    assert rel_l2(phi_g[s, s, s, 0], out["phi"][s, s, s, 0]) <= tol
+    assert rel_l2(d1["vel"], vel) <= tol
+    keys = orc.bin_keys(pos, N)
+    mism = np.nonzero(d1["cell"] != keys)[0]
+    t = pos[mism] * N
+    ambiguous = np.min(np.abs(t - np.round(t)), axis=1) < 1e-6
+    assert np.all(ambiguous), f"{len(mism)} cell mismatches, {int(np.sum(~ambiguous))} not at a face"
+    diff = np.abs(d1["pos"] - pos)
+    diff = np.minimum(diff, 1.0 - diff)
+    assert np.max(diff) * N <= (1e-9 if prec == "f64" else 1e-5)
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_pm_step_c1_uniform_forces(prec, tol):
+    """64^3 lattice, 32^3 uniform particles of mass 1 at rest: rho, phi and the gathered
+    force (read as the velocity after a unit kick with no drift)"""
+    N = 64
+    announce(f"pm_step C1 {prec}")
+    d0, d1, rho, phi = _run_step(N, prec, {"kind": "uniform", "np": 32 ** 3}, 1.0, 0.0, seed=1, mass=1.0)
+    out = _check_step(N, prec, d0, d1, rho, phi, 1.0, 0.0, 1.0, tol)
+    assert np.array_equal(d1["cell"], d0["cell"])
+    assert rel_l2(d1["vel"], out["acc"]) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
+def test_pm_step_lattice_gaussian_64(prec, tol):
+    """the kick-drift configuration at 64^3: grid + N(0, 0.3 cells), v ~ N(0, 0.01), dt 0.01,
+    mass 1/Np (mean density 1)"""
+    N, dt = 64, 0.01
+    announce(f"pm_step lattice 64 {prec}")
+    ic = {"kind": "lattice", "ng": N, "sigma_cells": 0.3, "sigma_v": 0.01}
+    d0, d1, rho, phi = _run_step(N, prec, ic, dt, dt, seed=2, mass=1.0 / N ** 3)
+    _check_step(N, prec, d0, d1, rho, phi, dt, dt, 1.0 / N ** 3, tol)
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("f64", 1e-10)])
+def test_pm_step_blobs_64(prec, tol):
+    """clustered Pareto blobs (sigma 2 cells, 20% uniform) at 64^3, mass 1/Np"""
+    N, dt = 64, 0.01
+    announce(f"pm_step blobs 64 {prec}")
+    ic = {"kind": "blobs", "np": N ** 3, "nblobs": 512, "sigma_v": 0.01}
+    d0, d1, rho, phi = _run_step(N, prec, ic, dt, dt, seed=3, mass=1.0 / N ** 3)
+    _check_step(N, prec, d0, d1, rho, phi, dt, dt, 1.0 / N ** 3, tol)
