@@ -71,6 +71,16 @@ int lf2mi_device_init(int device);
 /* hipDeviceSynchronize with status */
 int lf2mi_synchronize(void);
 
+/* device memory for callers that do not link the HIP runtime themselves (the C++ Field
+ * and Particles mirrors allocate through these, as Field::alloc does with new[] in
+ * LATfield2_Field_decl.hpp:696-712).  lf2mi_copy synchronises for host transfers. */
+#define LF2MI_COPY_H2D 0
+#define LF2MI_COPY_D2H 1
+#define LF2MI_COPY_D2D 2
+int lf2mi_device_alloc(void **ptr, int64_t bytes);
+int lf2mi_device_free(void *ptr);
+int lf2mi_copy(void *dst, const void *src, int64_t bytes, int kind, void *stream);
+
 /* Per-kernel timing (no counterpart in the reference, whose timer.hpp:64-177 and the
  * MPI_Wtime brackets of benchmarks_tests/benchmarks_particles.cpp:189-236 time stages
  * on the host).  When enabled, every kernel launch records a pair of HIP events on the
@@ -208,7 +218,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
  *   scalar-deposit weights; pos += dt_drift * vel, wrapped (:2028-2032) and re-keyed.
  * In F32 the new position is formed in FP64 from (cell, offset, vel) and re-anchored to
  * (cell, FP32 offset).  The container is re-bucketed afterwards.  vmax2 (host, may be
- * NULL) receives max |v|^2 after the kick (synchronises when non-NULL). */
+ * NULL) receives max |v|^2 after the kick (synchronises when non-NULL).  phi may be
+ * NULL when dt_kick == 0: a drift only (move_particles_simple, move_function.hpp:30). */
 int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick,
                             double dt_drift, double *vmax2, void *stream);
 

@@ -35,6 +35,9 @@ SIGNATURES = {
     "lf2mi_last_error": [],
     "lf2mi_device_init": [_i],
     "lf2mi_synchronize": [],
+    "lf2mi_device_alloc": [ctypes.POINTER(_vp), _i64],
+    "lf2mi_device_free": [_vp],
+    "lf2mi_copy": [_vp, _vp, _i64, _i, _vp],
     "lf2mi_profile_enable": [_i],
     "lf2mi_profile_reset": [],
     "lf2mi_profile_get": [_i, ctypes.c_char_p, _i, _dp, _i64p],

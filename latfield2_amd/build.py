@@ -18,6 +18,9 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblf2mi.so")
 SOURCES = ["field.hip", "fft.hip", "particles.hip"]
+BIN = os.path.join(PKG, "_bin")
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+CXX = os.environ.get("CXX_HOST", shutil.which("g++") or "g++")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -56,9 +59,35 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = True) -> list:
+    """C++ programs written against include/LATfield2_mi.hpp (the drop-in mirror of the
+    reference's API), each in double and in SINGLE precision, linked to liblf2mi.so."""
+    os.makedirs(BIN, exist_ok=True)
+    jobs = []
+    for src in sorted(f for f in os.listdir(CPP_TESTS) if f.endswith(".cpp")):
+        base = os.path.splitext(src)[0]
+        for tag, defs in (("f64", []), ("f32", ["-DSINGLE"])):
+            out = os.path.join(BIN, f"{base}_{tag}")
+            jobs.append([CXX, "-O2", "-std=c++17", "-Wall", *defs, "-I", os.path.join(ROOT, "include"),
+                         os.path.join(CPP_TESTS, src), "-L", PKG, "-llf2mi", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+
+    def run(cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ build failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
+        return cmd[-1]
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        outs = list(ex.map(run, jobs))
+    if verbose:
+        print(f"built {len(outs)} C++ drop-in programs in {BIN}")
+    return outs
+
+
 if __name__ == "__main__":
     try:
         build()
+        build_cpp_tests()
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -143,7 +143,8 @@ template <typename T> struct StridedArgs {
     int ncols;                  // valid columns per row
     const cplx<T> *tw;
     // Green fusion (MODE 1): element (col, e, batch) has kx = col / C, ky = e, kz = batch
-    const double *green_s;      // s[k] = N^2 (2 - 2 cos(2 pi k / N))
+    const double *green_s;      // s[k] = N^2 (2 - 2 cos(2 pi k / N)), FP64 table
+    const float *green_sf;      // the same table rounded to FP32 (FP32 path)
     double green_scale;
     int C;
 };
@@ -184,14 +185,19 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
         tile_fft<T, N, L, L, WG, S>(lds, A.tw);
     } else {
         tile_fft<T, N, L, L, WG, -1>(lds, A.tw);
-        const double skz = A.green_s[batch];
+        // Green's factor -scale / K^2, K^2 = s[kx] + s[ky] + s[kz], zero mode 0; in the
+        // transform's precision (FP32 tables for the FP32 path)
+        const T skz = sizeof(T) == 4 ? (T)A.green_sf[batch] : (T)A.green_s[batch];
+        const T gsc = (T)A.green_scale;
         for (int i = threadIdx.x; i < N * L; i += WG) {
             const int l = i % L, e = i / L;
             const int col = col0 + l;
             if (col < A.ncols) {
                 const int kx = col / A.C;
-                const double k2 = A.green_s[kx] + A.green_s[e] + skz;
-                const T g = (k2 > 0.0) ? (T)(-A.green_scale / k2) : T(0);
+                const T sx = sizeof(T) == 4 ? (T)A.green_sf[kx] : (T)A.green_s[kx];
+                const T sy = sizeof(T) == 4 ? (T)A.green_sf[e] : (T)A.green_s[e];
+                const T k2 = sx + sy + skz;
+                const T g = (k2 > T(0)) ? -gsc / k2 : T(0);
                 V v = lds[e * L + l];
                 v.x *= g;
                 v.y *= g;
@@ -350,6 +356,7 @@ struct lf2mi_fft_plan_s {
     int device;
     void *tw;        // N complex twiddles (dtype)
     double *green_s; // N doubles
+    float *green_sf; // N floats
     void *work;      // backward / Poisson workspace: N * N rows of wcols complex
     int64_t wcols;   // padded columns per workspace row
     int64_t bytes;
@@ -521,6 +528,7 @@ static int poisson(lf2mi_fft_plan_s *p, const void *rho, void *phi, double shift
     A.tw = (const cplx<T> *)p->tw;
     A.C = p->C;
     A.green_s = p->green_s;
+    A.green_sf = p->green_sf;
     A.green_scale = scale;
     // z forward along slot1
     A.in_es = A.out_es = wg.s1;
@@ -621,6 +629,17 @@ int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_f
     if ((e = hipMalloc(&p->tw, esz * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(twiddles)", __FILE__, __LINE__);
     if (!rc && (e = hipMalloc((void **)&p->green_s, sizeof(double) * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(green)", __FILE__, __LINE__);
     if (!rc && (e = hipMalloc(&p->work, wbytes)) != hipSuccess) rc = hip_fail(e, "hipMalloc(workspace)", __FILE__, __LINE__);
+    if (!rc && (e = hipMalloc((void **)&p->green_sf, sizeof(float) * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(green32)", __FILE__, __LINE__);
+    if (!rc) {
+        float *gsf = (float *)malloc(sizeof(float) * N);
+        if (!gsf) rc = fail(LF2MI_ENOMEM, "host allocation failed");
+        else {
+            for (int q = 0; q < N; ++q) gsf[q] = (float)gs[q];
+            if ((e = hipMemcpy(p->green_sf, gsf, sizeof(float) * N, hipMemcpyHostToDevice)) != hipSuccess)
+                rc = hip_fail(e, "hipMemcpy(green32)", __FILE__, __LINE__);
+            free(gsf);
+        }
+    }
     if (!rc && (e = hipMemcpy(p->tw, dtype == LF2MI_F32 ? (void *)twf : (void *)twd, esz * N, hipMemcpyHostToDevice)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy(twiddles)", __FILE__, __LINE__);
     if (!rc && (e = hipMemcpy(p->green_s, gs, sizeof(double) * N, hipMemcpyHostToDevice)) != hipSuccess)
@@ -631,11 +650,12 @@ int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_f
     if (rc) {
         hipFree(p->tw);
         hipFree(p->green_s);
+        hipFree(p->green_sf);
         hipFree(p->work);
         delete p;
         return rc;
     }
-    p->bytes = (int64_t)esz * N + (int64_t)sizeof(double) * N + wbytes;
+    p->bytes = (int64_t)esz * N + (int64_t)(sizeof(double) + sizeof(float)) * N + wbytes;
     *plan = p;
     return LF2MI_OK;
 }
@@ -644,6 +664,7 @@ int lf2mi_fft_plan_destroy(lf2mi_fft_plan p) {
     if (!p) return LF2MI_OK;
     hipFree(p->tw);
     hipFree(p->green_s);
+    hipFree(p->green_sf);
     hipFree(p->work);
     delete p;
     return LF2MI_OK;

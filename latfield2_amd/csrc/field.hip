@@ -256,6 +256,34 @@ int lf2mi_synchronize(void) {
     return LF2MI_OK;
 }
 
+int lf2mi_device_alloc(void **ptr, int64_t bytes) {
+    if (!ptr || bytes < 0) return fail(LF2MI_EINVAL, "lf2mi_device_alloc: bad argument");
+    *ptr = nullptr;
+    if (bytes == 0) return LF2MI_OK;
+    LF2MI_HIP(hipMalloc(ptr, (size_t)bytes));
+    return LF2MI_OK;
+}
+
+int lf2mi_device_free(void *ptr) {
+    if (ptr) LF2MI_HIP(hipFree(ptr));
+    return LF2MI_OK;
+}
+
+int lf2mi_copy(void *dst, const void *src, int64_t bytes, int kind, void *stream) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail(LF2MI_EINVAL, "lf2mi_copy: bad argument");
+    if (bytes == 0) return LF2MI_OK;
+    hipMemcpyKind k;
+    switch (kind) {
+        case LF2MI_COPY_H2D: k = hipMemcpyHostToDevice; break;
+        case LF2MI_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+        case LF2MI_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+        default: return fail(LF2MI_EINVAL, "lf2mi_copy: bad kind %d", kind);
+    }
+    LF2MI_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, as_stream(stream)));
+    if (kind != LF2MI_COPY_D2D) LF2MI_HIP(hipStreamSynchronize(as_stream(stream)));
+    return LF2MI_OK;
+}
+
 int lf2mi_profile_enable(int on) {
     g_prof_on = on != 0;
     return LF2MI_OK;

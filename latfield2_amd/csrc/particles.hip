@@ -87,13 +87,91 @@ __device__ __forceinline__ int agg_inc(int *counters, int key, bool active) {
     return slot;
 }
 
+// block-wide aggregated atomic increment (blockDim.x == BLK, every thread calls it):
+// lanes of the whole block with the same key share one global atomicAdd.  Particles are
+// processed in bucket order, so a block's keys are a handful of hot buckets; per-wave
+// aggregation alone leaves one atomic per wave on the same counter, which serialises in
+// L2 for clustered distributions.  Returns each lane's slot when want_slot.
+template <int BLK>
+struct BlockAgg {
+    int key[BLK];
+    int cnt[BLK];
+    int base[BLK];
+    int n;
+};
+
+template <int BLK>
+__device__ __forceinline__ int block_agg_inc(int *counters, int key, bool active, BlockAgg<BLK> &S) {
+    const int lane = lane_id();
+    if (threadIdx.x == 0) S.n = 0;
+    __syncthreads();
+    // per wave: groups of equal keys, each leader appends (key, count) to the block list
+    unsigned long long todo = __ballot(active);
+    int my_entry = -1, my_rank = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long grp = __ballot(active && key == lkey);
+        int e = 0;
+        if (lane == leader) {
+            e = atomicAdd(&S.n, 1);
+            S.key[e] = lkey;
+            S.cnt[e] = __popcll(grp);
+        }
+        e = __shfl(e, leader);
+        if (active && key == lkey) {
+            my_entry = e;
+            my_rank = __popcll(grp & ((1ull << lane) - 1ull));
+        }
+        todo &= ~grp;
+    }
+    __syncthreads();
+    // per entry: offset among earlier entries of the same key; the first entry of a key
+    // does the global atomic for the key's total
+    const int n = S.n;
+    int off = 0, first = -1, total = 0;
+    if ((int)threadIdx.x < n) {
+        const int k = S.key[threadIdx.x];
+        for (int e = 0; e < n; ++e) {
+            if (S.key[e] != k) continue;
+            if (first < 0) first = e;
+            if (e < (int)threadIdx.x) off += S.cnt[e];
+            total += S.cnt[e];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < n && first == (int)threadIdx.x) S.base[threadIdx.x] = atomicAdd(&counters[S.key[threadIdx.x]], total);
+    __syncthreads();
+    if ((int)threadIdx.x < n) S.cnt[threadIdx.x] = S.base[first] + off;  // reuse cnt as the entry's base
+    __syncthreads();
+    const int slot = my_entry >= 0 ? S.cnt[my_entry] + my_rank : 0;
+    __syncthreads();
+    return slot;
+}
+
 // ---------------------------------------------------------------- geometry helpers
 struct Geo {
     int N, B, NB;
     int64_t nbk;
+    int lgN, lgB;  // log2 of N and B when they are powers of two, else -1
 };
+// cell index x + N(y + N z) -> coordinates; shifts when N is a power of two
+__device__ __forceinline__ void decode_cell(int cell, const Geo &g, int &x, int &y, int &z) {
+    if (g.lgN >= 0) {
+        const int m = g.N - 1;
+        x = cell & m;
+        y = (cell >> g.lgN) & m;
+        z = cell >> (2 * g.lgN);
+    } else {
+        x = cell % g.N;
+        y = (cell / g.N) % g.N;
+        z = cell / (g.N * g.N);
+    }
+}
 __device__ __forceinline__ int bucket_of(int cell, const Geo &g) {
-    const int x = cell % g.N, y = (cell / g.N) % g.N, z = cell / (g.N * g.N);
+    int x, y, z;
+    decode_cell(cell, g, x, y, z);
+    if (g.lgB >= 0) return (x >> g.lgB) + g.NB * ((y >> g.lgB) + g.NB * (z >> g.lgB));
     return (x / g.B) + g.NB * ((y / g.B) + g.NB * (z / g.B));
 }
 __device__ __forceinline__ int wrapi(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
@@ -271,6 +349,29 @@ __device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total) {
     return incl - v;
 }
 
+// exclusive block scan with wave shuffles: two barriers (blockDim.x = SCAN_T = 256)
+__device__ __forceinline__ int block_excl_scan_fast(int v, int *wsum, int &total) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_T / 64; ++w) {
+        const int t = wsum[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
 __global__ void scan_local_kernel(const int *__restrict__ count, int64_t n, int *__restrict__ start,
                                   int *__restrict__ istart, int *__restrict__ bsum) {
     __shared__ int sh[SCAN_T];
@@ -356,14 +457,15 @@ __global__ void scan_add_kernel(int *__restrict__ start, int *__restrict__ istar
 }
 
 // ---------------------------------------------------------------- scatter (counting sort)
-__global__ void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g, const int *__restrict__ start,
-                               int *__restrict__ cursor) {
+__global__ __launch_bounds__(256) void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g,
+                                                      const int *__restrict__ start, int *__restrict__ cursor) {
+    __shared__ BlockAgg<256> agg;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < np;
     const int cell = in ? src.cell[i] : -1;
     const bool act = in && cell >= 0;
     const int b = act ? bucket_of(cell, g) : 0;
-    const int slot = agg_inc(cursor, b, act);
+    const int slot = block_agg_inc<256>(cursor, b, act, agg);
     if (!act) return;
     const int64_t j = (int64_t)start[b] + slot;
     dst.cell[j] = cell;
@@ -376,14 +478,12 @@ __global__ void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g, const in
 }
 
 // ---------------------------------------------------------------- items
-__device__ __forceinline__ int find_bucket(const int *__restrict__ istart, int64_t nbk, int item) {
-    int64_t lo = 0, hi = nbk;  // istart[lo] <= item < istart[hi]
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (istart[mid] <= item) lo = mid;
-        else hi = mid;
-    }
-    return (int)lo;
+// item -> bucket table, filled after every scan (a bucket of c particles owns
+// ceil(c / ITEM_P) consecutive items)
+__global__ void item_bucket_kernel(const int *__restrict__ istart, int64_t nbk, int *__restrict__ item_bucket) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbk) return;
+    for (int k = istart[b]; k < istart[b + 1]; ++k) item_bucket[k] = (int)b;
 }
 
 struct ItemRange {
@@ -392,9 +492,9 @@ struct ItemRange {
     int ox, oy, oz;  // brick origin (cells)
 };
 __device__ __forceinline__ bool item_range(int item, const int *__restrict__ start, const int *__restrict__ istart,
-                                           const Geo &g, ItemRange &r) {
+                                           const int *__restrict__ item_bucket, const Geo &g, ItemRange &r) {
     if (item >= istart[g.nbk]) return false;
-    const int b = find_bucket(istart, g.nbk, item);
+    const int b = item_bucket[item];
     const int k = item - istart[b];
     r.b = b;
     r.p0 = start[b] + k * ITEM_P;
@@ -405,133 +505,237 @@ __device__ __forceinline__ bool item_range(int item, const int *__restrict__ sta
     return true;
 }
 
-// ---------------------------------------------------------------- deposit (F32, LDS window)
-// Window nodes [o-1, o+B] per axis (W = B+2) for each of the C components.
-// KIND 0 scalar (C=1), 1 vector (C=3), 2 vecvec (C=6).  TGT 0 unfolded, 1 periodic.
+// ---------------------------------------------------------------- deposit (F32)
+// The reference accumulates, per cell, the contributions of the cell's particles to the
+// nodes around it (localCube[8] / vi[36] / tii[24] + tij[54], particles/projections.hpp:
+// 140-179, :1034-1122, :1421-1521) and then adds them to the field.  Here one workgroup
+// takes an item (a chunk of one brick's particles) and
+//   A. counting-sorts the chunk by cell in LDS (one int LDS atomic per particle);
+//   B. one thread per cell accumulates its particles into per-cell slots in registers,
+//      one component at a time, in a fixed order, and stores them to LDS;
+//   C. one thread per node sums the slots of the cells around it (fixed order) and adds
+//      the node value into the field with one global atomic (periodic wrap or the
+//      unfolded ghost layer).
+// Slots of a component: per axis the node offsets a cell's particles reach, {0, 1} on the
+// vertex grid and {-1, 0, +1} on the link grid (nodes at n + 1/2), so 8 (scalar, T_ii),
+// 12 (vector component) or 18 (T_ij) slots.
 template <int KIND> struct KindC;
-template <> struct KindC<0> { static constexpr int C = 1; };
-template <> struct KindC<1> { static constexpr int C = 3; };
-template <> struct KindC<2> { static constexpr int C = 6; };
+template <> struct KindC<0> { static constexpr int C = 1; static constexpr int SUBP = ITEM_P; static constexpr int NSMAX = 8; };
+template <> struct KindC<1> { static constexpr int C = 3; static constexpr int SUBP = ITEM_P / 2; static constexpr int NSMAX = 12; };
+template <> struct KindC<2> { static constexpr int C = 6; static constexpr int SUBP = ITEM_P / 2; static constexpr int NSMAX = 18; };
 
-struct AxisW {
-    int lo;      // node offset of the lower node relative to the cell (-1 or 0)
-    float w0, w1;
-};
-__device__ __forceinline__ AxisW vtx_w(float u) {  // vertex grid
-    AxisW a;
-    a.lo = 0;
-    a.w0 = 1.0f - u;
-    a.w1 = u;
-    return a;
-}
-__device__ __forceinline__ AxisW link_w(float u) {  // link grid, nodes at n + 1/2
-    AxisW a;
+// link-grid weights of one axis as a 3-vector over node offsets {-1, 0, +1}
+// (projections.hpp:1058-1071: the branch at the link midpoint)
+__device__ __forceinline__ void link3(float u, float &wm, float &w0, float &wp) {
     const float t = u - 0.5f;
     if (t < 0.0f) {
-        a.lo = -1;
-        a.w0 = -t;
-        a.w1 = 1.0f - a.w0;
+        const float a = -t;
+        wm = a;
+        w0 = 1.0f - a;
+        wp = 0.0f;
     } else {
-        a.lo = 0;
-        a.w1 = t;
-        a.w0 = 1.0f - a.w1;
+        wm = 0.0f;
+        w0 = 1.0f - t;
+        wp = t;
     }
-    return a;
 }
 
-template <typename ACC>
-__device__ __forceinline__ void lds_add(ACC *win, int idx, float v, double fxscale);
-template <>
-__device__ __forceinline__ void lds_add<float>(float *win, int idx, float v, double) {
-    atomicAdd(&win[idx], v);
-}
-template <>
-__device__ __forceinline__ void lds_add<unsigned long long>(unsigned long long *win, int idx, float v, double fxscale) {
-    const long long q = llrint((double)v * fxscale);
-    atomicAdd(&win[idx], (unsigned long long)q);
-}
-
-template <typename ACC>
-__device__ __forceinline__ void add8(ACC *win, int W, int comp, int lx, int ly, int lz, const AxisW &ax,
-                                     const AxisW &ay, const AxisW &az, float q, double fxscale) {
-    const int W3 = W * W * W;
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const float w = (i ? ax.w1 : ax.w0) * (j ? ay.w1 : ay.w0) * (k ? az.w1 : az.w0);
-                const int idx = comp * W3 + (lx + ax.lo + i + 1) + W * ((ly + ay.lo + j + 1) + W * (lz + az.lo + k + 1));
-                lds_add<ACC>(win, idx, q * w, fxscale);
-            }
-}
+// component c of kind KIND: stagger mask (bit d: link grid along axis d) and velocity factors
+template <int KIND, int COMP> struct Comp;
+template <> struct Comp<0, 0> { static constexpr int MASK = 0, VA = -1, VB = -1; };
+template <> struct Comp<1, 0> { static constexpr int MASK = 1, VA = 0, VB = -1; };
+template <> struct Comp<1, 1> { static constexpr int MASK = 2, VA = 1, VB = -1; };
+template <> struct Comp<1, 2> { static constexpr int MASK = 4, VA = 2, VB = -1; };
+template <> struct Comp<2, 0> { static constexpr int MASK = 0, VA = 0, VB = 0; };  // (0,0)
+template <> struct Comp<2, 1> { static constexpr int MASK = 3, VA = 0, VB = 1; };  // (0,1)
+template <> struct Comp<2, 2> { static constexpr int MASK = 5, VA = 0, VB = 2; };  // (0,2)
+template <> struct Comp<2, 3> { static constexpr int MASK = 0, VA = 1, VB = 1; };  // (1,1)
+template <> struct Comp<2, 4> { static constexpr int MASK = 6, VA = 1, VB = 2; };  // (1,2)
+template <> struct Comp<2, 5> { static constexpr int MASK = 0, VA = 2, VB = 2; };  // (2,2)
 
 struct DepArgs {
     Soa32 s;
     const int *start;
     const int *istart;
+    const int *item_bucket;
     Geo g;
-    float mass_nv;        // global mass * N^3 (mass / res^3)
-    float *field;         // native accumulation target
-    unsigned long long *fx; // fixed-point target (KIND 0 only)
-    double fxscale;       // 2^32 / (mass_nv_ref) for the fixed path
+    float mass_nv;           // global mass * N^3 (mass / res^3)
+    float *field;            // native accumulation target
+    unsigned long long *fx;  // fixed-point target (KIND 0 only)
+    double fxscale;          // 2^32 / mass_nv: fixed point counts dimensionless weights
     int h;
     int tgt;
 };
 
-template <int KIND, typename ACC>
-__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
-    constexpr int C = KindC<KIND>::C;
-    const int W = A.g.B + 2;
-    const int W3 = W * W * W;
-    __shared__ ACC win[C * (MAXB + 2) * (MAXB + 2) * (MAXB + 2)];
-    ItemRange r;
-    if (!item_range(blockIdx.x, A.start, A.istart, A.g, r)) return;
-    for (int i = threadIdx.x; i < C * W3; i += DEP_WG) win[i] = ACC(0);
-    __syncthreads();
-    const int N = A.g.N;
-    for (int p = r.p0 + threadIdx.x; p < r.p1; p += DEP_WG) {
-        const int cell = A.s.cell[p];
-        const int lx = cell % N - r.ox, ly = (cell / N) % N - r.oy, lz = cell / (N * N) - r.oz;
-        const float u[3] = {A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]};
-        const float m = A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv;
-        const AxisW v0 = vtx_w(u[0]), v1 = vtx_w(u[1]), v2 = vtx_w(u[2]);
-        if constexpr (KIND == 0) {
-            add8<ACC>(win, W, 0, lx, ly, lz, v0, v1, v2, m, A.fxscale);
-        } else {
-            const float vel[3] = {A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p]};
-            const AxisW l0 = link_w(u[0]), l1 = link_w(u[1]), l2 = link_w(u[2]);
-            if constexpr (KIND == 1) {
-                add8<ACC>(win, W, 0, lx, ly, lz, l0, v1, v2, m * vel[0], A.fxscale);
-                add8<ACC>(win, W, 1, lx, ly, lz, v0, l1, v2, m * vel[1], A.fxscale);
-                add8<ACC>(win, W, 2, lx, ly, lz, v0, v1, l2, m * vel[2], A.fxscale);
-            } else {
-                add8<ACC>(win, W, 0, lx, ly, lz, v0, v1, v2, m * vel[0] * vel[0], A.fxscale);
-                add8<ACC>(win, W, 3, lx, ly, lz, v0, v1, v2, m * vel[1] * vel[1], A.fxscale);
-                add8<ACC>(win, W, 5, lx, ly, lz, v0, v1, v2, m * vel[2] * vel[2], A.fxscale);
-                add8<ACC>(win, W, 1, lx, ly, lz, l0, l1, v2, m * vel[0] * vel[1], A.fxscale);
-                add8<ACC>(win, W, 2, lx, ly, lz, l0, v1, l2, m * vel[0] * vel[2], A.fxscale);
-                add8<ACC>(win, W, 4, lx, ly, lz, v0, l1, l2, m * vel[1] * vel[2], A.fxscale);
+template <typename ACC> struct AccOps;
+template <> struct AccOps<float> {
+    static __device__ __forceinline__ float term(float q, float w, double) { return q * w; }
+    static __device__ __forceinline__ void flush(const DepArgs &A, int64_t idx, float v) {
+        atomicAdd(&A.field[idx], v);
+    }
+};
+template <> struct AccOps<unsigned long long> {
+    // each particle's term is quantised on its own, so the integer sums do not depend on order
+    static __device__ __forceinline__ unsigned long long term(float q, float w, double fxscale) {
+        return (unsigned long long)llrint((double)(q * w) * fxscale);
+    }
+    static __device__ __forceinline__ void flush(const DepArgs &A, int64_t idx, unsigned long long v) {
+        atomicAdd(&A.fx[idx], v);
+    }
+};
+
+template <int KIND, int COMP, typename ACC>
+__device__ __forceinline__ void deposit_component(const DepArgs &A, const ItemRange &r, int NC, const int *cst,
+                                                  const float4 *sp, const float4 *sv, ACC *cube) {
+    constexpr int MASK = Comp<KIND, COMP>::MASK;
+    constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
+    constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
+    constexpr int NS = SX * SY * SZ;
+    const int B = A.g.B;
+    // B. per-cell slots
+    for (int c = threadIdx.x; c < NC; c += DEP_WG) {
+        ACC acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
+        const int j1 = cst[c + 1];
+        for (int j = cst[c]; j < j1; ++j) {
+            const float4 d = sp[j];
+            float q = d.w;
+            if constexpr (VA >= 0) {
+                const float4 v = sv[j];
+                const float va = VA == 0 ? v.x : (VA == 1 ? v.y : v.z);
+                q *= va;
+                if constexpr (VB >= 0) q *= (VB == 0 ? v.x : (VB == 1 ? v.y : v.z));
             }
+            float wx[3], wy[3], wz[3];
+            if constexpr (MASK & 1) link3(d.x, wx[0], wx[1], wx[2]);
+            else { wx[0] = 1.0f - d.x; wx[1] = d.x; }
+            if constexpr (MASK & 2) link3(d.y, wy[0], wy[1], wy[2]);
+            else { wy[0] = 1.0f - d.y; wy[1] = d.y; }
+            if constexpr (MASK & 4) link3(d.z, wz[0], wz[1], wz[2]);
+            else { wz[0] = 1.0f - d.z; wz[1] = d.z; }
+#pragma unroll
+            for (int kz = 0; kz < SZ; ++kz)
+#pragma unroll
+                for (int ky = 0; ky < SY; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < SX; ++kx)
+                        acc[kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
         }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) cube[c * NS + k] = acc[k];
     }
     __syncthreads();
-    // flush the window: node (o - 1 + w) per axis
-    for (int i = threadIdx.x; i < C * W3; i += DEP_WG) {
-        const ACC v = win[i];
+    // C. node gather: nodes with local coordinate in [lo, B] per axis, lo = -1 on link axes
+    const int lox = (MASK & 1) ? -1 : 0, loy = (MASK & 2) ? -1 : 0, loz = (MASK & 4) ? -1 : 0;
+    const int nx = B + 1 - lox, ny = B + 1 - loy, nz = B + 1 - loz;
+    const int N = A.g.N;
+    for (int i = threadIdx.x; i < nx * ny * nz; i += DEP_WG) {
+        const int ax = i % nx + lox, ay = (i / nx) % ny + loy, az = i / (nx * ny) + loz;
+        ACC v = ACC(0);
+        // slot offset o = node - cell in [lo_axis, 1]; cell = node - o must lie in [0, B)
+#pragma unroll
+        for (int oz = (MASK & 4) ? -1 : 0; oz <= 1; ++oz) {
+            const int cz = az - oz;
+            if (cz < 0 || cz >= B) continue;
+#pragma unroll
+            for (int oy = (MASK & 2) ? -1 : 0; oy <= 1; ++oy) {
+                const int cy = ay - oy;
+                if (cy < 0 || cy >= B) continue;
+#pragma unroll
+                for (int ox = (MASK & 1) ? -1 : 0; ox <= 1; ++ox) {
+                    const int cx = ax - ox;
+                    if (cx < 0 || cx >= B) continue;
+                    const int slot = (ox - ((MASK & 1) ? -1 : 0)) + SX * ((oy - ((MASK & 2) ? -1 : 0)) + SY * (oz - ((MASK & 4) ? -1 : 0)));
+                    v += cube[(cx + B * (cy + B * cz)) * NS + slot];
+                }
+            }
+        }
         if (v == ACC(0)) continue;
-        const int c = i / W3;
-        const int s = i % W3;
-        int gx = r.ox - 1 + s % W, gy = r.oy - 1 + (s / W) % W, gz = r.oz - 1 + s / (W * W);
+        int gx = r.ox + ax, gy = r.oy + ay, gz = r.oz + az;
         if (A.tgt == LF2MI_DEP_PERIODIC) {
             gx = wrapi(gx, N);
             gy = wrapi(gy, N);
             gz = wrapi(gz, N);
         }
-        const int64_t idx = site_index(N, A.h, gx, gy, gz) * C + c;
-        if constexpr (sizeof(ACC) == 4) atomicAdd(&A.field[idx], (float)v);
-        else atomicAdd(&A.fx[idx], (unsigned long long)v);
+        AccOps<ACC>::flush(A, site_index(N, A.h, gx, gy, gz) * KindC<KIND>::C + COMP, v);
+    }
+    __syncthreads();
+}
+
+// grid: one workgroup per item for KIND 0, two per item (halves of ITEM_P/2) otherwise
+template <int KIND, typename ACC>
+__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
+    constexpr int SUBP = KindC<KIND>::SUBP;
+    constexpr int PER = SUBP / DEP_WG;
+    constexpr int SPLIT = ITEM_P / SUBP;
+    __shared__ int cnt[MAXB * MAXB * MAXB];
+    __shared__ int cst[MAXB * MAXB * MAXB + 1];
+    __shared__ int scan_sh[DEP_WG];
+    __shared__ float4 sp[SUBP];
+    __shared__ float4 sv[KIND > 0 ? SUBP : 1];
+    __shared__ ACC cube[MAXB * MAXB * MAXB * KindC<KIND>::NSMAX];
+    ItemRange r;
+    if (!item_range(blockIdx.x / SPLIT, A.start, A.istart, A.item_bucket, A.g, r)) return;
+    const int p0 = r.p0 + (int)(blockIdx.x % SPLIT) * SUBP;
+    const int p1 = min(p0 + SUBP, r.p1);
+    if (p0 >= p1) return;
+    const int B = A.g.B, N = A.g.N;
+    const int NC = B * B * B;
+    for (int i = threadIdx.x; i < NC; i += DEP_WG) cnt[i] = 0;
+    __syncthreads();
+    // A. load and rank by cell
+    int lc[PER], rk[PER];
+    float4 pd[PER], pv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int p = p0 + threadIdx.x + k * DEP_WG;
+        lc[k] = -1;
+        if (p < p1) {
+            const int cell = A.s.cell[p];
+            int lx, ly, lz;
+            decode_cell(cell, A.g, lx, ly, lz);
+            lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
+            pd[k] = make_float4(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p],
+                                A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv);
+            if constexpr (KIND > 0) pv[k] = make_float4(A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p], 0.0f);
+            rk[k] = atomicAdd(&cnt[lc[k]], 1);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of cnt over NC <= 512 cells: two cells per thread
+    {
+        const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
+        const int a0 = c0 < NC ? cnt[c0] : 0, a1 = c1 < NC ? cnt[c1] : 0;
+        int total;
+        const int ex = block_excl_scan_fast(a0 + a1, scan_sh, total);
+        if (c0 < NC) cst[c0] = ex;
+        if (c1 < NC) cst[c1] = ex + a0;
+        if (threadIdx.x == 0) cst[NC] = total;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (lc[k] >= 0) {
+            const int j = cst[lc[k]] + rk[k];
+            sp[j] = pd[k];
+            if constexpr (KIND > 0) sv[j] = pv[k];
+        }
+    }
+    __syncthreads();
+    if constexpr (KIND == 0) {
+        deposit_component<0, 0, ACC>(A, r, NC, cst, sp, sv, cube);
+    } else if constexpr (KIND == 1) {
+        deposit_component<1, 0, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<1, 1, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<1, 2, ACC>(A, r, NC, cst, sp, sv, cube);
+    } else {
+        deposit_component<2, 0, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 1, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 2, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 3, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 4, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 5, ACC>(A, r, NC, cst, sp, sv, cube);
     }
 }
 
@@ -549,6 +753,7 @@ struct GkdArgs {
     Soa32 s;
     const int *start;
     const int *istart;
+    const int *item_bucket;
     Geo g;
     const float *phi;
     int h;
@@ -558,87 +763,173 @@ struct GkdArgs {
     unsigned int *vmax2;    // bits of max |v|^2 (non-negative float)
 };
 
+// kick with the gathered acceleration, drift in FP64 cell units, re-anchor and store:
+// X = (c + u) + fl(dt v) N, then the sequential periodic wrap of Particles.hpp:2028-2032
+// and re-binning to (cell, FP32 offset).  Returns the new cell.
+__device__ __forceinline__ int kick_drift_store(const GkdArgs &A, int p, int cx, int cy, int cz, const float u[3],
+                                                const float vin[3], const float a[3], float &vmax) {
+    const int N = A.g.N;
+    float v[3];
+    int c3[3] = {cx, cy, cz};
+    float o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        v[d] = vin[d] + A.dt_kick * a[d];
+        double X = add_rn((double)c3[d], (double)u[d]);
+        X = add_rn(X, mul_rn(A.dt_drift, (double)v[d]) * (double)N);
+        if (X < 0.0) X = add_rn(X, (double)N);
+        if (X >= (double)N) X = sub_rn(X, (double)N);
+        // beyond the reference's range (|dt v| >= box) wrap fully so the cell stays valid
+        if (X < 0.0 || X >= (double)N) X -= (double)N * floor(X / (double)N);
+        const double fl = floor(X);
+        int cn = (int)fl;
+        if (cn >= N) cn = N - 1;
+        if (cn < 0) cn = 0;
+        float on = (float)(X - fl);
+        if (!(on < 1.0f)) on = 0x1.fffffep-1f;
+        c3[d] = cn;
+        o[d] = on;
+    }
+    const int newcell = c3[0] + N * (c3[1] + N * c3[2]);
+    A.s.cell[p] = newcell;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        A.s.off[d][p] = o[d];
+        A.s.vel[d][p] = v[d];
+    }
+    vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    return newcell;
+}
+
+// CIC gather of F = -grad phi straight from phi (32 values around the cell)
+__device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, int cz, const float u[3], float a[3]) {
+    const int N = A.g.N, h = A.h;
+    const int xs[4] = {wrapi(cx - 1, N), cx, wrapi(cx + 1, N), wrapi(cx + 2, N)};
+    const int ys[4] = {wrapi(cy - 1, N), cy, wrapi(cy + 1, N), wrapi(cy + 2, N)};
+    const int zs[4] = {wrapi(cz - 1, N), cz, wrapi(cz + 1, N), wrapi(cz + 2, N)};
+    auto P = [&](int x, int y, int z) { return A.phi[site_index(N, h, x, y, z)]; };
+    float core[2][2][2], xw[2][2][2], yw[2][2][2], zw[2][2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                core[k][j][i] = P(xs[1 + i], ys[1 + j], zs[1 + k]);
+                xw[k][j][i] = P(xs[i ? 3 : 0], ys[1 + j], zs[1 + k]);  // i selects x-1 / x+2
+                yw[k][j][i] = P(xs[1 + i], ys[j ? 3 : 0], zs[1 + k]);  // j selects y-1 / y+2
+                zw[k][j][i] = P(xs[1 + i], ys[1 + j], zs[k ? 3 : 0]);  // k selects z-1 / z+2
+            }
+    const float hn = 0.5f * (float)N;
+    a[0] = a[1] = a[2] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const float w = (i ? u[0] : 1.0f - u[0]) * (j ? u[1] : 1.0f - u[1]) * (k ? u[2] : 1.0f - u[2]);
+                // neighbours of node (i,j,k) along each axis, from the core or the wings
+                const float xp = i ? xw[k][j][1] : core[k][j][1], xm = i ? core[k][j][0] : xw[k][j][0];
+                const float yp = j ? yw[k][1][i] : core[k][1][i], ym = j ? core[k][0][i] : yw[k][0][i];
+                const float zp = k ? zw[1][j][i] : core[1][j][i], zm = k ? core[0][j][i] : zw[0][j][i];
+                a[0] += w * (-(xp - xm) * hn);
+                a[1] += w * (-(yp - ym) * hn);
+                a[2] += w * (-(zp - zm) * hn);
+            }
+}
+
+static constexpr int GKD_DIRECT_MAX = 64;
+
 __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     constexpr int PW = MAXB + 3;  // phi window [o-1, o+B+1]
     constexpr int FW = MAXB + 1;  // force nodes [o, o+B]
     __shared__ float phw[PW * PW * PW];
     __shared__ float frc[3][FW * FW * FW];
     __shared__ unsigned int red;
+    __shared__ BlockAgg<DEP_WG> agg;
     ItemRange r;
-    if (!item_range(blockIdx.x, A.start, A.istart, A.g, r)) return;
+    if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
     const int N = A.g.N, B = A.g.B;
-    const int pw = B + 3, fw = B + 1;
     if (threadIdx.x == 0) red = 0u;
-    for (int i = threadIdx.x; i < pw * pw * pw; i += DEP_WG) {
-        const int gx = wrapi(r.ox - 1 + i % pw, N);
-        const int gy = wrapi(r.oy - 1 + (i / pw) % pw, N);
-        const int gz = wrapi(r.oz - 1 + i / (pw * pw), N);
-        phw[i] = A.phi[site_index(N, A.h, gx, gy, gz)];
-    }
-    __syncthreads();
-    const float hn = 0.5f * (float)N;
-    for (int i = threadIdx.x; i < fw * fw * fw; i += DEP_WG) {
-        const int x = i % fw + 1, y = (i / fw) % fw + 1, z = i / (fw * fw) + 1;
-        const int c = x + pw * (y + pw * z);
-        frc[0][i] = -(phw[c + 1] - phw[c - 1]) * hn;
-        frc[1][i] = -(phw[c + pw] - phw[c - pw]) * hn;
-        frc[2][i] = -(phw[c + pw * pw] - phw[c - pw * pw]) * hn;
-    }
-    __syncthreads();
     float vmax = 0.0f;
-    for (int base = r.p0; base < r.p1; base += DEP_WG) {
-        const int p = base + threadIdx.x;
-        const bool act = p < r.p1;
-        int newcell = 0;
-        if (act) {
-            const int cell = A.s.cell[p];
-            const int cx = cell % N, cy = (cell / N) % N, cz = cell / (N * N);
-            const int lx = cx - r.ox, ly = cy - r.oy, lz = cz - r.oz;
-            const float u[3] = {A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]};
-            float a[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const float w = (i ? u[0] : 1.0f - u[0]) * (j ? u[1] : 1.0f - u[1]) * (k ? u[2] : 1.0f - u[2]);
-                        const int n = (lx + i) + fw * ((ly + j) + fw * (lz + k));
-                        a[0] += w * frc[0][n];
-                        a[1] += w * frc[1][n];
-                        a[2] += w * frc[2][n];
-                    }
-            float v[3];
-            int c3[3] = {cx, cy, cz};
-            float o[3];
-            for (int d = 0; d < 3; ++d) {
-                v[d] = A.s.vel[d][p] + A.dt_kick * a[d];
-                // drift in FP64 cell units: X = (c + u) + fl(dt v) N, then the sequential
-                // periodic wrap of Particles.hpp:2028-2032 and re-binning
-                double X = add_rn((double)c3[d], (double)u[d]);
-                X = add_rn(X, mul_rn(A.dt_drift, (double)v[d]) * (double)N);
-                if (X < 0.0) X = add_rn(X, (double)N);
-                if (X >= (double)N) X = sub_rn(X, (double)N);
-                // beyond the reference's range (|dt v| >= box) wrap fully so the cell stays valid
-                if (X < 0.0 || X >= (double)N) X -= (double)N * floor(X / (double)N);
-                const double fl = floor(X);
-                int cn = (int)fl;
-                if (cn >= N) cn = N - 1;
-                if (cn < 0) cn = 0;
-                float on = (float)(X - fl);
-                if (!(on < 1.0f)) on = 0x1.fffffep-1f;
-                c3[d] = cn;
-                o[d] = on;
+    if (!A.phi || r.p1 - r.p0 < GKD_DIRECT_MAX) {
+        // sparse item (no window) or drift only (no potential: a = 0)
+        for (int base = r.p0; base < r.p1; base += DEP_WG) {
+            const int p = base + threadIdx.x;
+            const bool act = p < r.p1;
+            int newcell = 0;
+            if (act) {
+                int cx, cy, cz;
+                decode_cell(A.s.cell[p], A.g, cx, cy, cz);
+                const float u[3] = {A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]};
+                const float vin[3] = {A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p]};
+                float a[3] = {0.f, 0.f, 0.f};
+                if (A.phi) gather_direct(A, cx, cy, cz, u, a);
+                newcell = kick_drift_store(A, p, cx, cy, cz, u, vin, a, vmax);
             }
-            newcell = c3[0] + N * (c3[1] + N * c3[2]);
-            A.s.cell[p] = newcell;
-            for (int d = 0; d < 3; ++d) {
-                A.s.off[d][p] = o[d];
-                A.s.vel[d][p] = v[d];
-            }
-            vmax = fmaxf(vmax, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            block_agg_inc<DEP_WG>(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act, agg);
         }
-        agg_inc(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act);
+    } else {
+        const int pw = B + 3, fw = B + 1;
+        // first batch of particle data in flight while the window loads
+        int p = r.p0 + threadIdx.x;
+        bool act = p < r.p1;
+        int cell = 0;
+        float u[3] = {0.f, 0.f, 0.f}, vin[3] = {0.f, 0.f, 0.f};
+        if (act) {
+            cell = A.s.cell[p];
+            u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
+            vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
+        }
+        for (int i = threadIdx.x; i < pw * pw * pw; i += DEP_WG) {
+            const int gx = wrapi(r.ox - 1 + i % pw, N);
+            const int gy = wrapi(r.oy - 1 + (i / pw) % pw, N);
+            const int gz = wrapi(r.oz - 1 + i / (pw * pw), N);
+            phw[i] = A.phi[site_index(N, A.h, gx, gy, gz)];
+        }
+        __syncthreads();
+        const float hn = 0.5f * (float)N;
+        for (int i = threadIdx.x; i < fw * fw * fw; i += DEP_WG) {
+            const int x = i % fw + 1, y = (i / fw) % fw + 1, z = i / (fw * fw) + 1;
+            const int c = x + pw * (y + pw * z);
+            frc[0][i] = -(phw[c + 1] - phw[c - 1]) * hn;
+            frc[1][i] = -(phw[c + pw] - phw[c - pw]) * hn;
+            frc[2][i] = -(phw[c + pw * pw] - phw[c - pw * pw]) * hn;
+        }
+        __syncthreads();
+        for (int base = r.p0; base < r.p1; base += DEP_WG) {
+            if (base != r.p0) {
+                p = base + threadIdx.x;
+                act = p < r.p1;
+                if (act) {
+                    cell = A.s.cell[p];
+                    u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
+                    vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
+                }
+            }
+            int newcell = 0;
+            if (act) {
+                int cx, cy, cz;
+                decode_cell(cell, A.g, cx, cy, cz);
+                const int lx = cx - r.ox, ly = cy - r.oy, lz = cz - r.oz;
+                float a[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            const float w = (i ? u[0] : 1.0f - u[0]) * (j ? u[1] : 1.0f - u[1]) * (k ? u[2] : 1.0f - u[2]);
+                            const int n = (lx + i) + fw * ((ly + j) + fw * (lz + k));
+                            a[0] += w * frc[0][n];
+                            a[1] += w * frc[1][n];
+                            a[2] += w * frc[2][n];
+                        }
+                newcell = kick_drift_store(A, p, cx, cy, cz, u, vin, a, vmax);
+            }
+            block_agg_inc<DEP_WG>(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act, agg);
+        }
     }
     if (A.vmax2) {
         atomicMax(&red, __float_as_uint(vmax));
@@ -744,6 +1035,7 @@ __global__ void gkd_f64_kernel(Gkd64Args A) {
     }
     const double hn = 0.5 * N;
     double a[3] = {0.0, 0.0, 0.0};
+    if (A.phi)
     for (int k = 0; k < 2; ++k)
         for (int j = 0; j < 2; ++j)
             for (int i = 0; i < 2; ++i) {
@@ -819,7 +1111,7 @@ struct lf2mi_parts_s {
     lf2mi::Soa32 b32[2];
     int cur;
     lf2mi::Soa64 b64;
-    int *bstart, *bcount, *istart, *bsum;
+    int *bstart, *bcount, *istart, *bsum, *item_bucket;
     unsigned long long *fx;
     int64_t fx_n;
     unsigned int *vmax_f32;
@@ -861,6 +1153,8 @@ static int scan_buckets(lf2mi_parts_s *p, hipStream_t st) {
     scan_sums_kernel<<<1, 1024, 0, st>>>(p->bsum, nblocks, p->bstart, p->istart, n);
     LF2MI_LAUNCHED();
     scan_add_kernel<<<nblocks, SCAN_T, 0, st>>>(p->bstart, p->istart, p->bsum, n);
+    LF2MI_LAUNCHED();
+    item_bucket_kernel<<<grid1(n), 256, 0, st>>>(p->istart, n, p->item_bucket);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
@@ -934,6 +1228,8 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
     p->g.B = B;
     p->g.NB = N / B;
     p->g.nbk = (int64_t)p->g.NB * p->g.NB * p->g.NB;
+    p->g.lgN = is_pow2(N) ? __builtin_ctz((unsigned)N) : -1;
+    p->g.lgB = is_pow2(B) ? __builtin_ctz((unsigned)B) : -1;
     const size_t cap = (size_t)std::max<int64_t>(capacity, 1);
     int rc = LF2MI_OK;
     auto A = [&](void **ptr, size_t bytes) {
@@ -957,6 +1253,7 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         A((void **)&p->istart, nb * sizeof(int));
         A((void **)&p->bcount, nb * sizeof(int));
         A((void **)&p->bsum, 2 * 4096 * sizeof(int));
+        A((void **)&p->item_bucket, (size_t)(p->g.nbk + cap / ITEM_P + 1) * sizeof(int));
         A((void **)&p->vmax_f32, sizeof(unsigned int));
     } else {
         Soa64 &s = p->b64;
@@ -1260,6 +1557,7 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     A.s = p->b32[p->cur];
     A.start = p->bstart;
     A.istart = p->istart;
+    A.item_bucket = p->item_bucket;
     A.g = p->g;
     A.mass_nv = (float)((p->flags & LF2MI_PARTS_MASS) ? nv : p->mass_global * nv);
     A.field = (float *)field;
@@ -1297,15 +1595,16 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     }
     LF2MI_PROF(kind == 0 ? "deposit_f32_scalar" : (kind == 1 ? "deposit_f32_vector" : "deposit_f32_vecvec"), st);
     if (kind == 0) deposit_f32_kernel<0, float><<<g, DEP_WG, 0, st>>>(A);
-    else if (kind == 1) deposit_f32_kernel<1, float><<<g, DEP_WG, 0, st>>>(A);
-    else deposit_f32_kernel<2, float><<<g, DEP_WG, 0, st>>>(A);
+    else if (kind == 1) deposit_f32_kernel<1, float><<<2 * g, DEP_WG, 0, st>>>(A);
+    else deposit_f32_kernel<2, float><<<2 * g, DEP_WG, 0, st>>>(A);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
 
 int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick, double dt_drift, double *vmax2,
                             void *stream) {
-    if (!p || !phi || halo < 0) return fail(LF2MI_EINVAL, "lf2mi_gather_kick_drift: bad argument");
+    if (!p || halo < 0) return fail(LF2MI_EINVAL, "lf2mi_gather_kick_drift: bad argument");
+    if (!phi && dt_kick != 0.0) return fail(LF2MI_EINVAL, "lf2mi_gather_kick_drift: a kick needs the potential field");
     hipStream_t st = as_stream(stream);
     if (p->np == 0) {
         if (vmax2) *vmax2 = 0.0;
@@ -1340,6 +1639,7 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
     A.s = p->b32[p->cur];
     A.start = p->bstart;
     A.istart = p->istart;
+    A.item_bucket = p->item_bucket;
     A.g = p->g;
     A.phi = (const float *)phi;
     A.h = halo;

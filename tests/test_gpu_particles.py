@@ -89,7 +89,9 @@ def _oracle_deposits(pos, vel, mass, N, h, fold):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
 @pytest.mark.parametrize("N,h,n,clustered,masses", [(16, 1, 5000, False, False), (32, 2, 40000, True, True),
-                                                     (24, 1, 3000, True, False)])
+                                                     (24, 1, 3000, True, False),
+                                                     (64, 1, 6000, False, False),     # ~12 per brick: direct path
+                                                     (16, 1, 120000, True, True)])    # ~15k per brick: many items
 @pytest.mark.parametrize("mode", ["unfolded", "unfolded+comm", "periodic"])
 def test_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mode):
     announce(f"deposits {prec} N={N} h={h} n={n} clustered={clustered} masses={masses} mode={mode}")
@@ -166,10 +168,11 @@ def test_drift_cells_bit_exact_on_identical_velocities(prec):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
-def test_gather_kick_matches_oracle(prec, tol):
-    N = 32
-    announce(f"gather+kick {prec} N={N}")
-    pos, vel, _ = _random_particles(30000, 21, clustered=True)
+@pytest.mark.parametrize("N,n", [(32, 30000), (64, 4000), (16, 100000)])
+def test_gather_kick_matches_oracle(prec, tol, N, n):
+    """window path (dense bricks, several items per brick) and direct path (sparse bricks)"""
+    announce(f"gather+kick {prec} N={N} n={n}")
+    pos, vel, _ = _random_particles(n, 21 + N, clustered=True)
     vel *= 0.01
     parts, _ = _upload(N, pos, vel, prec)
     d0 = _by_id(parts.download())

@@ -117,6 +117,28 @@ def test_poisson_solver_round_trip():
     assert err.max() <= 1e-11
 
 
+def test_float32_floor_of_the_poisson_check():
+    """examples/poissonSolver.cpp:153-158 asks for 1e-6 under SINGLE.  With the exact FP64
+    potential rounded to float32 and the 7-point Laplacian evaluated in float32 (as the
+    SINGLE build does) the check already reports ~7e-5: 1e-6 is out of reach of any
+    float32 FFT, so the FP32 drop-in program is held to 1e-3 and the FP32 path is pinned
+    against the FP64 oracle instead."""
+    N, res2 = 64, 0.25
+    c = np.arange(N)
+    d = (0.5 + c - N / 2) ** 2
+    rho = np.exp(-(d[:, None, None] + d[None, :, None] + d[None, None, :]) / (N * N / 9.0)) + 0.1
+    k = np.arange(N)
+    cs = np.cos(2 * np.pi * k / N)
+    lam = 2.0 * (cs[:, None, None] + cs[None, :, None] + cs[None, None, :N // 2 + 1] - 3.0) / res2
+    lam[0, 0, 0] = 1.0
+    ph = np.fft.rfftn(rho) / lam
+    ph[0, 0, 0] = 0
+    phi = (np.fft.irfftn(ph, s=(N, N, N), axes=(0, 1, 2)) * N ** 3).astype(np.float32)
+    lap = sum((np.roll(phi, -1, a) - 2 * phi + np.roll(phi, 1, a)) / np.float32(res2) for a in range(3))
+    err = np.abs(rho.astype(np.float32) - (np.float32(rho.mean()) + lap / np.float32(N ** 3))) / rho
+    assert 1e-5 < err.max() < 1e-3
+
+
 def test_fft_roundtrip_is_n3_identity():
     """examples/fft_c2c_r2c_sametime.cpp:90-99, with ghosts on both lattices"""
     N = 12  # not a power of two: exercises the direct-DFT branch
