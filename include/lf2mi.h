@@ -230,6 +230,41 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
 int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int halo,
                   double dt_kick, double dt_drift, int acc, void *stream);
 
+/* ------------------------------------------------------------ differentiable PM */
+/* No counterpart in the reference: the adjoint of nsteps of its PM cycle
+ * (scalarProjectionCIC_project particles/projections.hpp:76-300, the Poisson solve of
+ * examples/poissonSolver.cpp:99-111, the gather of updateVel Particles.hpp:825-828 and
+ * the drift of moveParticles :2028-2032), for callers that fit initial conditions.
+ * Forward, box length 1, periodic, particle order kept:
+ *   rho = CIC deposit of q = mass N^3 per particle; phi = G rho, G = -1/K^2, G(0) = 0;
+ *   F = -grad phi (central differences); v += dt * gather(F); x += dt * v (wrapped);
+ *   loss = sum_cells (rho(x_final) - target)^2 over the N^3 interior.
+ * The starting state of every step is checkpointed on the device; the backward pass
+ * recomputes each step's fields and runs the hand-written adjoint: the gather's
+ * transpose is a CIC deposit, the central difference is antisymmetric, G is symmetric
+ * (the same FFT solve), and the CIC weights' position derivatives give the direct
+ * terms.  dtype (F32/F64) is the precision of the whole path. */
+typedef struct lf2mi_diffpm_s *lf2mi_diffpm;
+
+int lf2mi_diffpm_create(int N, int64_t np, int nsteps, double mass, double dt, int dtype,
+                        lf2mi_diffpm *out);
+int lf2mi_diffpm_destroy(lf2mi_diffpm m);
+/* device bytes held by the handle (checkpoints, fields, adjoints, FFT workspace) */
+int lf2mi_diffpm_bytes(lf2mi_diffpm m, int64_t *bytes);
+/* pos0, vel0: np*3 (x,y,z per particle) in box units, device, io_dtype F32 or F64;
+ * vel0 may be NULL (zero).  target: N^3 device values, x fastest, io_dtype, may be
+ * NULL (zero).  rho_out (may be NULL) receives the final density, N^3, x fastest.
+ * loss (host, may be NULL) receives L and synchronises the stream. */
+int lf2mi_diffpm_forward(lf2mi_diffpm m, const void *pos0, const void *vel0,
+                         const void *target, int io_dtype, void *rho_out, double *loss,
+                         void *stream);
+/* dL/dpos0 and dL/dvel0 (np*3 each, io_dtype of the last forward; grad_vel may be NULL) */
+int lf2mi_diffpm_backward(lf2mi_diffpm m, void *grad_pos, void *grad_vel, void *stream);
+/* positions (box units) and velocities at the start of step `step` (0..nsteps; nsteps
+ * is the final state) of the last forward; either output may be NULL */
+int lf2mi_diffpm_state(lf2mi_diffpm m, int step, void *pos, void *vel, int io_dtype,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,12 @@ SIGNATURES = {
     "lf2mi_deposit": [_vp, _i, _vp, _i, _i, _i, _vp],
     "lf2mi_gather_kick_drift": [_vp, _vp, _i, _d, _d, _dp, _vp],
     "lf2mi_pm_step": [_vp, _vp, _vp, _vp, _i, _d, _d, _i, _vp],
+    "lf2mi_diffpm_create": [_i, _i64, _i, _d, _d, _i, ctypes.POINTER(_vp)],
+    "lf2mi_diffpm_destroy": [_vp],
+    "lf2mi_diffpm_bytes": [_vp, _i64p],
+    "lf2mi_diffpm_forward": [_vp, _vp, _vp, _vp, _i, _vp, _dp, _vp],
+    "lf2mi_diffpm_backward": [_vp, _vp, _vp, _vp],
+    "lf2mi_diffpm_state": [_vp, _i, _vp, _vp, _i, _vp],
 }
 
 _lib = None

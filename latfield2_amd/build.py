@@ -17,7 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblf2mi.so")
-SOURCES = ["field.hip", "fft.hip", "particles.hip"]
+SOURCES = ["field.hip", "fft.hip", "particles.hip", "diffpm.hip"]
 BIN = os.path.join(PKG, "_bin")
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
 CXX = os.environ.get("CXX_HOST", shutil.which("g++") or "g++")

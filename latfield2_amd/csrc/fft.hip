@@ -75,13 +75,30 @@ template <int R, int S, typename V> __device__ __forceinline__ void dft(V *a) {
     else dft8<S>(a);
 }
 
-// ---------------------------------------------------------------- Stockham passes in LDS
-// Tile of L lines of length N; element e of line l lives at lds[e*ES + l].
-// One pass of radix R after sub-transforms of size NS: butterfly j reads e = j + k N/R,
-// multiplies by W_N^{(j mod NS) k N/(NS R)}, does a radix-R DFT and writes
-// e = (j/NS) NS R + (j mod NS) + k NS.
-template <typename T, int N, int L, int ES, int WG, int S, int R, int NS>
-__device__ __forceinline__ void stockham_pass(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
+// ---------------------------------------------------------------- Stockham passes
+// Tile of L lines of length N.  Radix plan: 8 = [8], 16 = [4,4], 32 = [8,4], 64 = [8,8],
+// 128 = [8,4,4], 256 = [8,8,4], 512 = [8,8,8], 1024 = [8,8,4,4].
+constexpr int fft_np(int N) { return N == 8 ? 1 : (N <= 64 ? 2 : (N <= 512 ? 3 : 4)); }
+constexpr int fft_radix(int N, int p) {
+    return N == 16 ? 4
+         : N == 32 ? (p == 0 ? 8 : 4)
+         : N == 128 ? (p == 0 ? 8 : 4)
+         : N == 256 ? (p < 2 ? 8 : 4)
+         : N == 1024 ? (p < 2 ? 8 : 4)
+         : 8;
+}
+constexpr int fft_ns(int N, int p) { return p == 0 ? 1 : fft_ns(N, p - 1) * fft_radix(N, p - 1); }
+
+// One radix-R pass after sub-transforms of size NS (Stockham auto-sort): butterfly j of
+// line l reads element e = j + k N/R, multiplies by W_N^{(j mod NS) k N/(NS R)}, does a
+// radix-R DFT and writes element (j/NS) NS R + (j mod NS) + k NS.  Inputs come from
+// load(l, e), outputs go to store(l, e, v): registers <- HBM on the first pass,
+// HBM <- registers on the last (both use e = j + k N/R), LDS in between.  BAR puts a
+// barrier between the reads and the writes (needed when both touch the same LDS tile).
+// MAP 0 maps threads line-fastest (coalesced across lines: the y/z passes); MAP 1 maps
+// them j-fastest (coalesced along a line: the x pass).
+template <typename T, int N, int L, int WG, int S, int R, int NS, int MAP, bool BAR, class Load, class Store>
+__device__ __forceinline__ void pass_g(const cplx<T> *__restrict__ tw, Load &&load, Store &&store) {
     using V = cplx<T>;
     constexpr int NBF = L * (N / R);
     constexpr int PER = (NBF + WG - 1) / WG;
@@ -90,17 +107,19 @@ __device__ __forceinline__ void stockham_pass(cplx<T> *lds, const cplx<T> *__res
     for (int b = 0; b < PER; ++b) {
         const int bf = threadIdx.x + b * WG;
         if (NBF % WG == 0 || bf < NBF) {
-            const int l = bf % L, j = bf / L;
+            const int l = MAP == 0 ? bf % L : bf / (N / R);
+            const int j = MAP == 0 ? bf / L : bf % (N / R);
 #pragma unroll
-            for (int k = 0; k < R; ++k) a[b][k] = lds[(j + k * (N / R)) * ES + l];
+            for (int k = 0; k < R; ++k) a[b][k] = load(l, j + k * (N / R));
         }
     }
-    __syncthreads();
+    if constexpr (BAR) __syncthreads();
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
         const int bf = threadIdx.x + b * WG;
         if (NBF % WG == 0 || bf < NBF) {
-            const int l = bf % L, j = bf / L;
+            const int l = MAP == 0 ? bf % L : bf / (N / R);
+            const int j = MAP == 0 ? bf / L : bf % (N / R);
             const int m = j % NS;
             if constexpr (NS > 1) {
 #pragma unroll
@@ -113,24 +132,37 @@ __device__ __forceinline__ void stockham_pass(cplx<T> *lds, const cplx<T> *__res
             dft<R, S>(a[b]);
             const int base = (j / NS) * NS * R + m;
 #pragma unroll
-            for (int k = 0; k < R; ++k) lds[(base + k * NS) * ES + l] = a[b][k];
+            for (int k = 0; k < R; ++k) store(l, base + k * NS, a[b][k]);
         }
     }
-    __syncthreads();
 }
 
-template <typename T, int N, int L, int ES, int WG, int S>
-__device__ __forceinline__ void tile_fft(cplx<T> *lds, const cplx<T> *__restrict__ tw) {
-#define LF2_PASS(R, NS) stockham_pass<T, N, L, ES, WG, S, R, NS>(lds, tw)
-    if constexpr (N == 8) { LF2_PASS(8, 1); }
-    else if constexpr (N == 16) { LF2_PASS(4, 1); LF2_PASS(4, 4); }
-    else if constexpr (N == 32) { LF2_PASS(8, 1); LF2_PASS(4, 8); }
-    else if constexpr (N == 64) { LF2_PASS(8, 1); LF2_PASS(8, 8); }
-    else if constexpr (N == 128) { LF2_PASS(8, 1); LF2_PASS(4, 8); LF2_PASS(4, 32); }
-    else if constexpr (N == 256) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(4, 64); }
-    else if constexpr (N == 512) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(8, 64); }
-    else if constexpr (N == 1024) { LF2_PASS(8, 1); LF2_PASS(8, 8); LF2_PASS(4, 64); LF2_PASS(4, 256); }
-#undef LF2_PASS
+// Whole transform of a tile.  FIRST_LDS / LAST_LDS say whether load0 reads and storeLast
+// writes the LDS tile (lds[e*ES + l]); otherwise they are HBM accessors.
+template <typename T, int N, int L, int ES, int WG, int S, int MAP, bool FIRST_LDS, bool LAST_LDS, class Load0,
+          class StoreLast>
+__device__ __forceinline__ void tile_fft_io(cplx<T> *lds, const cplx<T> *__restrict__ tw, Load0 &&load0,
+                                            StoreLast &&storeLast) {
+    using V = cplx<T>;
+    auto ld = [lds](int l, int e) { return lds[e * ES + l]; };
+    auto st = [lds](int l, int e, V v) { lds[e * ES + l] = v; };
+    constexpr int NP = fft_np(N);
+    if constexpr (NP == 1) {
+        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAP, FIRST_LDS && LAST_LDS>(tw, load0, storeLast);
+    } else {
+        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAP, FIRST_LDS>(tw, load0, st);
+        __syncthreads();
+        if constexpr (NP >= 3) {
+            pass_g<T, N, L, WG, S, fft_radix(N, 1), fft_ns(N, 1), MAP, true>(tw, ld, st);
+            __syncthreads();
+        }
+        if constexpr (NP >= 4) {
+            pass_g<T, N, L, WG, S, fft_radix(N, 2), fft_ns(N, 2), MAP, true>(tw, ld, st);
+            __syncthreads();
+        }
+        pass_g<T, N, L, WG, S, fft_radix(N, NP - 1), fft_ns(N, NP - 1), MAP, LAST_LDS>(tw, ld, storeLast);
+    }
+    if constexpr (LAST_LDS) __syncthreads();
 }
 
 // ---------------------------------------------------------------- kernel arguments
@@ -164,7 +196,7 @@ template <typename T> struct XArgs {
 };
 
 // ---------------------------------------------------------------- strided passes (y, z)
-// grid: x = column tiles of L, y = batches.  MODE 0: one transform of sign S.
+// grid: x = column tiles of L, y = batches.  MODE 0: one transform of sign S, HBM to HBM.
 // MODE 1: forward transform, Green multiply, inverse transform (fused Poisson y pass).
 template <typename T, int N, int L, int WG, int S, int MODE>
 __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
@@ -174,43 +206,38 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
     const int col0 = blockIdx.x * L;
     const int64_t batch = blockIdx.y;
     const V *in = A.in + A.in_col0 + batch * A.in_bs + col0;
-    for (int i = threadIdx.x; i < N * L; i += WG) {
-        const int l = i % L, e = i / L;
-        V v = mk<T>(0, 0);
-        if (col0 + l < A.ncols) v = in[(int64_t)e * A.in_es + l];
-        lds[e * L + l] = v;
-    }
-    __syncthreads();
+    V *out = A.out + A.out_col0 + batch * A.out_bs + col0;
+    const int ncols = A.ncols - col0;
+    auto gload = [in, ncols, &A](int l, int e) {
+        return l < ncols ? in[(int64_t)e * A.in_es + l] : mk<T>(0, 0);
+    };
+    auto gstore = [out, ncols, &A](int l, int e, V v) {
+        if (l < ncols) out[(int64_t)e * A.out_es + l] = v;
+    };
     if constexpr (MODE == 0) {
-        tile_fft<T, N, L, L, WG, S>(lds, A.tw);
+        tile_fft_io<T, N, L, L, WG, S, 0, false, false>(lds, A.tw, gload, gstore);
     } else {
-        tile_fft<T, N, L, L, WG, -1>(lds, A.tw);
-        // Green's factor -scale / K^2, K^2 = s[kx] + s[ky] + s[kz], zero mode 0; in the
-        // transform's precision (FP32 tables for the FP32 path)
+        // Green's factor -scale / K^2, K^2 = s[kx] + s[ky] + s[kz], zero mode 0, in the
+        // transform's precision (FP32 tables for the FP32 path); applied as the forward
+        // transform's last pass writes the tile
         const T skz = sizeof(T) == 4 ? (T)A.green_sf[batch] : (T)A.green_s[batch];
         const T gsc = (T)A.green_scale;
-        for (int i = threadIdx.x; i < N * L; i += WG) {
-            const int l = i % L, e = i / L;
-            const int col = col0 + l;
-            if (col < A.ncols) {
-                const int kx = col / A.C;
-                const T sx = sizeof(T) == 4 ? (T)A.green_sf[kx] : (T)A.green_s[kx];
-                const T sy = sizeof(T) == 4 ? (T)A.green_sf[e] : (T)A.green_s[e];
-                const T k2 = sx + sy + skz;
-                const T g = (k2 > T(0)) ? -gsc / k2 : T(0);
-                V v = lds[e * L + l];
-                v.x *= g;
-                v.y *= g;
-                lds[e * L + l] = v;
-            }
-        }
-        __syncthreads();
-        tile_fft<T, N, L, L, WG, +1>(lds, A.tw);
-    }
-    V *out = A.out + A.out_col0 + batch * A.out_bs + col0;
-    for (int i = threadIdx.x; i < N * L; i += WG) {
-        const int l = i % L, e = i / L;
-        if (col0 + l < A.ncols) out[(int64_t)e * A.out_es + l] = lds[e * L + l];
+        const int C = A.C;
+        auto green_store = [lds, skz, gsc, col0, C, &A](int l, int e, V v) {
+            // padding lines past the last column (kx > N/2) hold zeros; keep their table
+            // index inside the N-entry table
+            const int kx = min((col0 + l) / C, N / 2);
+            const T sx = sizeof(T) == 4 ? (T)A.green_sf[kx] : (T)A.green_s[kx];
+            const T sy = sizeof(T) == 4 ? (T)A.green_sf[e] : (T)A.green_s[e];
+            const T k2 = sx + sy + skz;
+            const T g = (k2 > T(0)) ? -gsc / k2 : T(0);
+            v.x *= g;
+            v.y *= g;
+            lds[e * L + l] = v;
+        };
+        auto lload = [lds](int l, int e) { return lds[e * L + l]; };
+        tile_fft_io<T, N, L, L, WG, -1, 0, false, true>(lds, A.tw, gload, green_store);
+        tile_fft_io<T, N, L, L, WG, +1, 0, true, false>(lds, A.tw, lload, gstore);
     }
 }
 
@@ -230,24 +257,20 @@ __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
     const int rh = A.rh;
     const int64_t G = N + 2 * rh;
     const T *real = (const T *)A.real_in;
-    const int NC = N * C;
-    // load: (p, ab, n, c) with (n, c) fastest -> contiguous within a row
-    for (int i = threadIdx.x; i < P * 2 * NC; i += WG) {
-        const int q = i % NC;
-        const int rest = i / NC;
-        const int ab = rest & 1, p = rest >> 1;
+    const T shift = A.shift;
+    // first pass straight from the two real rows of line (p, c)
+    auto rload = [&](int l, int n) {
+        const int p = l / C, c = l - p * C;
         const int64_t pr = pair0 + p;
-        if (pr >= npairs) continue;
-        const int64_t row = 2 * pr + ab;
+        if (p >= P || pr >= npairs) return mk<T>(0, 0);
+        const int64_t row = 2 * pr;
         const int y = (int)(row % N), z = (int)(row / N);
-        const int n = q / C, c = q % C;
-        const int64_t src = (((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C) + q;
-        const T v = real[src] - A.shift;
-        T *slot = reinterpret_cast<T *>(&lds[n * ES + p * C + c]);
-        slot[ab] = v;
-    }
-    __syncthreads();
-    tile_fft<T, N, L, ES, WG, -1>(lds, A.tw);
+        const int64_t src = ((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C + (int64_t)n * C + c;
+        // rows y and y+1 of the same plane (N even) are G*C apart
+        return mk<T>(real[src] - shift, real[src + G * C] - shift);
+    };
+    auto lstore = [lds](int l, int e, V v) { lds[e * ES + l] = v; };
+    tile_fft_io<T, N, L, ES, WG, -1, 1, false, true>(lds, A.tw, rload, lstore);
     // split the two half spectra: A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = -i (Z[k] - conj Z[N-k]) / 2
     constexpr int H = N / 2 + 1;
     const int HC = H * C;
@@ -316,23 +339,23 @@ __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
         }
     }
     __syncthreads();
-    tile_fft<T, N, L, ES, WG, +1>(lds, A.tw);
     const int rh = A.rh;
     const int64_t G = N + 2 * rh;
-    const int NC = N * C;
-    for (int i = threadIdx.x; i < P * 2 * NC; i += WG) {
-        const int q = i % NC;
-        const int rest = i / NC;
-        const int ab = rest & 1, p = rest >> 1;
+    T *real = A.real_out;
+    const T post = A.post;
+    auto lload = [lds](int l, int e) { return lds[e * ES + l]; };
+    // last pass straight into the two real rows of line (p, c)
+    auto rstore = [&](int l, int n, V v) {
+        const int p = l / C, c = l - p * C;
         const int64_t pr = pair0 + p;
-        if (pr >= npairs) continue;
-        const int64_t row = 2 * pr + ab;
+        if (p >= P || pr >= npairs) return;
+        const int64_t row = 2 * pr;
         const int y = (int)(row % N), z = (int)(row / N);
-        const int n = q / C, c = q % C;
-        const V v = lds[n * ES + p * C + c];
-        const int64_t dst = (((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C) + q;
-        A.real_out[dst] = (ab ? v.y : v.x) * A.post;
-    }
+        const int64_t dst = ((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C + (int64_t)n * C + c;
+        real[dst] = v.x * post;
+        real[dst + G * C] = v.y * post;
+    };
+    tile_fft_io<T, N, L, ES, WG, +1, 1, true, false>(lds, A.tw, lload, rstore);
 }
 
 // ---------------------------------------------------------------- launch configuration

@@ -71,6 +71,13 @@ def lib():
             "orc_drift": [_dp, _dp, _i64, ctypes.c_double],
             "orc_pm_step": [_dp, _dp, _dp, ctypes.c_double, _i64, ctypes.c_int, ctypes.c_double,
                             ctypes.c_double, _dp, _dp, _dp, _dp],
+            "orc_cic_density": [_dp, _i64, ctypes.c_int, ctypes.c_double, _dp],
+            "orc_diffpm_forward": [_dp, _dp, _i64, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, _dp],
+            "orc_diffpm_loss": [_dp, _dp, _i64, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int, _dp, _dp],
+            "orc_diffpm_loss_grad": [_dp, _dp, _i64, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, _dp, _dp, _dp, _dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -253,3 +260,43 @@ def pm_step(pos, vel, N, dt_kick, dt_drift, mass=None, mass_global=1.0, want_acc
                              dt_drift, _p(rho), _p(phi), _p(acc), _p(st)), "orc_pm_step")
     names = ["deposit", "fold", "mean", "fft_fwd", "green", "fft_bwd", "halo_force", "gather_kick_drift"]
     return {"rho": rho, "phi": phi, "acc": acc, "stage_ms": dict(zip(names, st.tolist()))}
+
+
+# ----------------------------------------------------------------------------- differentiable PM
+def cic_density(pos, N, mass):
+    """periodic CIC density (mass * N^3 per particle) on [z, y, x]"""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros((N, N, N))
+    _check(lib().orc_cic_density(_p(pos), pos.shape[0], N, mass, _p(out)), "orc_cic_density")
+    return out
+
+
+def diffpm_forward(pos, vel, N, mass, dt, nsteps):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    _check(lib().orc_diffpm_forward(_p(pos), _p(vel), pos.shape[0], N, mass, dt, nsteps, None),
+           "orc_diffpm_forward")
+    return pos, vel
+
+
+def diffpm_loss(pos0, vel0, N, mass, dt, nsteps, target):
+    pos0 = np.ascontiguousarray(pos0, dtype=np.float64)
+    vel0 = np.ascontiguousarray(vel0, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    L = ctypes.c_double(0.0)
+    _check(lib().orc_diffpm_loss(_p(pos0), _p(vel0), pos0.shape[0], N, mass, dt, nsteps, _p(target),
+                                 ctypes.byref(L)), "orc_diffpm_loss")
+    return L.value
+
+
+def diffpm_loss_grad(pos0, vel0, N, mass, dt, nsteps, target):
+    """(loss, dL/dpos0, dL/dvel0) in box units by the FP64 adjoint"""
+    pos0 = np.ascontiguousarray(pos0, dtype=np.float64)
+    vel0 = np.ascontiguousarray(vel0, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    L = ctypes.c_double(0.0)
+    gp = np.zeros_like(pos0)
+    gv = np.zeros_like(vel0)
+    _check(lib().orc_diffpm_loss_grad(_p(pos0), _p(vel0), pos0.shape[0], N, mass, dt, nsteps, _p(target),
+                                      ctypes.byref(L), _p(gp), _p(gv)), "orc_diffpm_loss_grad")
+    return L.value, gp, gv

@@ -796,3 +796,257 @@ out:
     free(F);
     return rc;
 }
+
+/* ---------------------------------------------------------------------------
+ * Differentiable PM: forward with checkpoints and the hand-derived adjoint.
+ * Grids here are periodic N^3 arrays without ghosts, index x + N(y + N z).
+ * ------------------------------------------------------------------------- */
+static inline void cic_cell(double p, int N, int *c, double *u) {
+    const double X = p * N;
+    const double f = floor(X);
+    *u = X - f;
+    int ci = (int)f % N;
+    if (ci < 0) ci += N;
+    *c = ci;
+}
+
+static inline i64 gidx(int N, int x, int y, int z) {
+    x = (x % N + N) % N;
+    y = (y % N + N) % N;
+    z = (z % N + N) % N;
+    return x + (i64)N * (y + (i64)N * z);
+}
+
+/* value and gradient (per cell unit, d/dX with X = pos*N) of the CIC interpolant of g */
+static double cic_interp_grad(const double *g, int N, const double *pos, double grad[3]) {
+    int c[3];
+    double u[3];
+    for (int d = 0; d < 3; ++d) cic_cell(pos[d], N, &c[d], &u[d]);
+    double val = 0.0;
+    if (grad) grad[0] = grad[1] = grad[2] = 0.0;
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const double wx = i ? u[0] : 1 - u[0], wy = j ? u[1] : 1 - u[1], wz = k ? u[2] : 1 - u[2];
+                const double gx = i ? 1.0 : -1.0, gy = j ? 1.0 : -1.0, gz = k ? 1.0 : -1.0;
+                const double v = g[gidx(N, c[0] + i, c[1] + j, c[2] + k)];
+                val += wx * wy * wz * v;
+                if (grad) {
+                    grad[0] += gx * wy * wz * v;
+                    grad[1] += wx * gy * wz * v;
+                    grad[2] += wx * wy * gz * v;
+                }
+            }
+    return val;
+}
+
+static void cic_add(double *g, int N, const double *pos, double q) {
+    int c[3];
+    double u[3];
+    for (int d = 0; d < 3; ++d) cic_cell(pos[d], N, &c[d], &u[d]);
+    for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const double w = (i ? u[0] : 1 - u[0]) * (j ? u[1] : 1 - u[1]) * (k ? u[2] : 1 - u[2]);
+                g[gidx(N, c[0] + i, c[1] + j, c[2] + k)] += q * w;
+            }
+}
+
+int orc_cic_density(const double *pos, i64 np, int N, double mass, double *out) {
+    if (!pos || !out || np < 0 || N < 2) return ORC_EBADARG;
+    memset(out, 0, sizeof(double) * (size_t)N * N * N);
+    const double q = mass * (double)N * N * N;
+    for (i64 p = 0; p < np; ++p) cic_add(out, N, pos + 3 * p, q);
+    return 0;
+}
+
+/* phi = G rho: mean removed, -1/K^2 per mode, zero mode 0, normalised (the PM Poisson solve).
+ * G is symmetric, so it is also its own adjoint. */
+static int poisson_periodic(const double *rho, double *phi, int N) {
+    const i64 n3 = (i64)N * N * N;
+    double *f = (double *)calloc((size_t)(N + 2) * (N + 2) * (N + 2), sizeof(double));
+    if (!f) return ORC_ENOMEM;
+    double mean = 0.0;
+    for (i64 i = 0; i < n3; ++i) mean += rho[i];
+    mean /= (double)n3;
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) f[sidx(N, 1, x, y, z)] = rho[gidx(N, x, y, z)] - mean;
+    const int H = N / 2 + 1;
+    double *k = (double *)malloc(sizeof(double) * 2 * (size_t)H * N * N);
+    if (!k) { free(f); return ORC_ENOMEM; }
+    int rc = orc_fft_r2c(f, k, N, 1, 1, 0);
+    if (!rc) rc = orc_poisson_green(k, N, 1, 0, 1.0 / N, 1.0 / (double)n3);
+    if (!rc) rc = orc_fft_c2r(k, f, N, 1, 1, 0);
+    if (!rc)
+        for (int z = 0; z < N; ++z)
+            for (int y = 0; y < N; ++y)
+                for (int x = 0; x < N; ++x) phi[gidx(N, x, y, z)] = f[sidx(N, 1, x, y, z)];
+    free(k);
+    free(f);
+    return rc;
+}
+
+/* F_d = -(phi(n+e_d) - phi(n-e_d)) N/2, three N^3 arrays */
+static void force_periodic(const double *phi, double *F, int N) {
+    const i64 n3 = (i64)N * N * N;
+    const double hn = 0.5 * N;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                const i64 i = gidx(N, x, y, z);
+                F[i] = -(phi[gidx(N, x + 1, y, z)] - phi[gidx(N, x - 1, y, z)]) * hn;
+                F[n3 + i] = -(phi[gidx(N, x, y + 1, z)] - phi[gidx(N, x, y - 1, z)]) * hn;
+                F[2 * n3 + i] = -(phi[gidx(N, x, y, z + 1)] - phi[gidx(N, x, y, z - 1)]) * hn;
+            }
+}
+
+static int step_forces(const double *pos, i64 np, int N, double mass, double *rho, double *phi, double *F) {
+    orc_cic_density(pos, np, N, mass, rho);
+    int rc = poisson_periodic(rho, phi, N);
+    if (rc) return rc;
+    force_periodic(phi, F, N);
+    return 0;
+}
+
+static void wrap01(double *x) {
+    if (*x < 0) *x += 1.0;
+    if (*x >= 1.0) *x -= 1.0;
+    if (*x < 0 || *x >= 1.0) *x -= floor(*x);
+}
+
+int orc_diffpm_forward(double *pos, double *vel, i64 np, int N, double mass, double dt, int nsteps, double *ckpt) {
+    if (!pos || !vel || np < 0 || N < 2 || nsteps < 0) return ORC_EBADARG;
+    const i64 n3 = (i64)N * N * N;
+    double *rho = (double *)malloc(sizeof(double) * (size_t)n3);
+    double *phi = (double *)malloc(sizeof(double) * (size_t)n3);
+    double *F = (double *)malloc(sizeof(double) * 3 * (size_t)n3);
+    if (!rho || !phi || !F) { free(rho); free(phi); free(F); return ORC_ENOMEM; }
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        if (ckpt) {
+            memcpy(ckpt + (i64)s * 6 * np, pos, sizeof(double) * 3 * (size_t)np);
+            memcpy(ckpt + (i64)s * 6 * np + 3 * np, vel, sizeof(double) * 3 * (size_t)np);
+        }
+        rc = step_forces(pos, np, N, mass, rho, phi, F);
+        if (rc) break;
+#pragma omp parallel for schedule(static)
+        for (i64 p = 0; p < np; ++p)
+            for (int d = 0; d < 3; ++d) {
+                const double a = cic_interp_grad(F + (i64)d * n3, N, pos + 3 * p, NULL);
+                vel[3 * p + d] += dt * a;
+            }
+#pragma omp parallel for schedule(static)
+        for (i64 p = 0; p < np; ++p)
+            for (int d = 0; d < 3; ++d) {
+                pos[3 * p + d] += dt * vel[3 * p + d];
+                wrap01(&pos[3 * p + d]);
+            }
+    }
+    free(rho);
+    free(phi);
+    free(F);
+    return rc;
+}
+
+static double density_loss(const double *pos, i64 np, int N, double mass, const double *target, double *resid) {
+    const i64 n3 = (i64)N * N * N;
+    orc_cic_density(pos, np, N, mass, resid);
+    double L = 0.0;
+    for (i64 i = 0; i < n3; ++i) {
+        resid[i] -= target[i];
+        L += resid[i] * resid[i];
+    }
+    return L;
+}
+
+int orc_diffpm_loss(const double *pos0, const double *vel0, i64 np, int N, double mass, double dt, int nsteps,
+                    const double *target, double *loss) {
+    if (!pos0 || !vel0 || !target || !loss) return ORC_EBADARG;
+    double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+    double *vel = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+    double *res = (double *)malloc(sizeof(double) * (size_t)N * N * N);
+    if (!pos || !vel || !res) { free(pos); free(vel); free(res); return ORC_ENOMEM; }
+    memcpy(pos, pos0, sizeof(double) * 3 * (size_t)np);
+    memcpy(vel, vel0, sizeof(double) * 3 * (size_t)np);
+    int rc = orc_diffpm_forward(pos, vel, np, N, mass, dt, nsteps, NULL);
+    if (!rc) *loss = density_loss(pos, np, N, mass, target, res);
+    free(pos);
+    free(vel);
+    free(res);
+    return rc;
+}
+
+int orc_diffpm_loss_grad(const double *pos0, const double *vel0, i64 np, int N, double mass, double dt, int nsteps,
+                         const double *target, double *loss, double *grad_pos, double *grad_vel) {
+    if (!pos0 || !vel0 || !target || !loss || !grad_pos || np < 0 || N < 2 || nsteps < 0) return ORC_EBADARG;
+    const i64 n3 = (i64)N * N * N;
+    const double q = mass * (double)n3;
+    double *ck = (double *)malloc(sizeof(double) * 6 * (size_t)np * (size_t)(nsteps > 0 ? nsteps : 1));
+    double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+    double *vel = (double *)malloc(sizeof(double) * 3 * (size_t)np);
+    double *lx = grad_pos;
+    double *lv = (double *)calloc(3 * (size_t)np, sizeof(double));
+    double *g = (double *)malloc(sizeof(double) * 5 * (size_t)n3);    /* resid/rho, phi, rhobar, phibar + spare */
+    double *F = (double *)malloc(sizeof(double) * 3 * (size_t)n3);
+    double *Fb = (double *)malloc(sizeof(double) * 3 * (size_t)n3);
+    if (!ck || !pos || !vel || !lv || !g || !F || !Fb) {
+        free(ck); free(pos); free(vel); free(lv); free(g); free(F); free(Fb);
+        return ORC_ENOMEM;
+    }
+    double *rho = g, *phi = g + n3, *rhob = g + 2 * n3, *phib = g + 3 * n3;
+    memcpy(pos, pos0, sizeof(double) * 3 * (size_t)np);
+    memcpy(vel, vel0, sizeof(double) * 3 * (size_t)np);
+    int rc = orc_diffpm_forward(pos, vel, np, N, mass, dt, nsteps, ck);
+    if (rc) goto out;
+    /* dL/dx_final: the position derivative of the deposit against 2 (rho - target) */
+    *loss = density_loss(pos, np, N, mass, target, rho);
+    for (i64 i = 0; i < n3; ++i) rho[i] *= 2.0;
+#pragma omp parallel for schedule(static)
+    for (i64 p = 0; p < np; ++p) {
+        double gr[3];
+        cic_interp_grad(rho, N, pos + 3 * p, gr);
+        for (int d = 0; d < 3; ++d) lx[3 * p + d] = q * N * gr[d];
+    }
+    for (int s = nsteps - 1; s >= 0; --s) {
+        const double *xs = ck + (i64)s * 6 * np;
+        /* x_{s+1} = x_s + dt v_{s+1}: the velocity adjoint collects dt * lambda_x */
+        for (i64 i = 0; i < 3 * np; ++i) lv[i] += dt * lx[i];
+        /* mu = dt * lambda_v is the adjoint of the acceleration a_s(x_s) */
+        if ((rc = step_forces(xs, np, N, mass, rho, phi, F))) goto out;
+        memset(Fb, 0, sizeof(double) * 3 * (size_t)n3);
+        for (i64 p = 0; p < np; ++p) {
+            const double *x = xs + 3 * p;
+            for (int d = 0; d < 3; ++d) {
+                const double mu = dt * lv[3 * p + d];
+                double gr[3];
+                /* direct term: the interpolation weights move with the particle */
+                cic_interp_grad(F + (i64)d * n3, N, x, gr);
+                for (int e = 0; e < 3; ++e) lx[3 * p + e] += mu * N * gr[e];
+                /* field term: a = I(x) F, so F-bar = I(x)^T mu (a CIC deposit of mu) */
+                cic_add(Fb + (i64)d * n3, N, x, mu);
+            }
+        }
+        /* F_d = -D_d phi with D_d antisymmetric: phi-bar = sum_d D_d F-bar_d */
+        const double hn = 0.5 * N;
+        for (int z = 0; z < N; ++z)
+            for (int y = 0; y < N; ++y)
+                for (int x = 0; x < N; ++x)
+                    phib[gidx(N, x, y, z)] = (Fb[gidx(N, x + 1, y, z)] - Fb[gidx(N, x - 1, y, z)]) * hn +
+                                             (Fb[n3 + gidx(N, x, y + 1, z)] - Fb[n3 + gidx(N, x, y - 1, z)]) * hn +
+                                             (Fb[2 * n3 + gidx(N, x, y, z + 1)] - Fb[2 * n3 + gidx(N, x, y, z - 1)]) * hn;
+        /* rho-bar = G^T phi-bar = G phi-bar; then the deposit's position derivative */
+        if ((rc = poisson_periodic(phib, rhob, N))) goto out;
+#pragma omp parallel for schedule(static)
+        for (i64 p = 0; p < np; ++p) {
+            double gr[3];
+            cic_interp_grad(rhob, N, xs + 3 * p, gr);
+            for (int d = 0; d < 3; ++d) lx[3 * p + d] += q * N * gr[d];
+        }
+    }
+    if (grad_vel) memcpy(grad_vel, lv, sizeof(double) * 3 * (size_t)np);
+out:
+    free(ck); free(pos); free(vel); free(lv); free(g); free(F); free(Fb);
+    return rc;
+}

@@ -119,6 +119,34 @@ int orc_pm_step(double *pos, double *vel, const double *mass, double mass_global
                 orc_i64 np, int N, double dt_kick, double dt_drift,
                 double *rho, double *phi, double *acc, double *stage_ms);
 
+/* ---------------------------------------------------------------------------
+ * Differentiable PM (no counterpart in the reference; the reference gives the
+ * forward pieces).  Periodic CIC deposit of mass m (density m N^3 per particle),
+ * Poisson phi = -K^-2 (rho - mean) with the zero mode removed, F = -grad phi by
+ * central differences, CIC gather.  One leapfrog step in kick-drift form:
+ *   v += dt a(x);  x += dt v;  x wrapped into [0,1).
+ * Loss after nsteps: L = sum_cells (rho(x_final) - target)^2 on the N^3 interior.
+ * ------------------------------------------------------------------------- */
+/* periodic CIC deposit of q_p = m N^3 onto out[N^3] (x fastest), cleared first */
+int orc_cic_density(const double *pos, orc_i64 np, int N, double mass, double *out);
+
+/* nsteps of the leapfrog in place; ckpt (may be NULL) receives pos and vel at the start
+ * of every step: ckpt[s*6*np + ...] = pos (np*3) then vel (np*3) */
+int orc_diffpm_forward(double *pos, double *vel, orc_i64 np, int N, double mass, double dt,
+                       int nsteps, double *ckpt);
+
+/* L and dL/dpos0, dL/dvel0 (box units) by the adjoint of the forward above:
+ * backward through the checkpointed steps with the transposed gather (deposit),
+ * the transposed central difference, the symmetric Green's operator and the position
+ * derivatives of the CIC weights.  grad_vel may be NULL. */
+int orc_diffpm_loss_grad(const double *pos0, const double *vel0, orc_i64 np, int N, double mass,
+                         double dt, int nsteps, const double *target, double *loss,
+                         double *grad_pos, double *grad_vel);
+
+/* the loss alone (forward only), for finite differences */
+int orc_diffpm_loss(const double *pos0, const double *vel0, orc_i64 np, int N, double mass,
+                    double dt, int nsteps, const double *target, double *loss);
+
 #ifdef __cplusplus
 }
 #endif
