@@ -280,7 +280,8 @@ class Particles:
 
     def gather_kick_drift(self, phi: Field, dt_kick: float, dt_drift: float, want_vmax: bool = False):
         vm = ctypes.c_double(0.0)
-        call("lf2mi_gather_kick_drift", self.h, _ptr(phi.t), phi.halo, float(dt_kick), float(dt_drift),
+        call("lf2mi_gather_kick_drift", self.h, None if phi is None else _ptr(phi.t), 1 if phi is None else phi.halo,
+             float(dt_kick), float(dt_drift),
              ctypes.byref(vm) if want_vmax else None, _stream())
         return vm.value if want_vmax else None
 

@@ -464,17 +464,30 @@ __global__ __launch_bounds__(256) void scatter_kernel(int64_t np, Soa32 src, Soa
     const bool in = i < np;
     const int cell = in ? src.cell[i] : -1;
     const bool act = in && cell >= 0;
+    // the particle's data is in flight while the slot is claimed
+    float o[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, m = 0.f;
+    int64_t id = 0;
+    if (act) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            o[d] = src.off[d][i];
+            v[d] = src.vel[d][i];
+        }
+        if (src.mass) m = src.mass[i];
+        if (src.id) id = src.id[i];
+    }
     const int b = act ? bucket_of(cell, g) : 0;
     const int slot = block_agg_inc<256>(cursor, b, act, agg);
     if (!act) return;
     const int64_t j = (int64_t)start[b] + slot;
     dst.cell[j] = cell;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        dst.off[d][j] = src.off[d][i];
-        dst.vel[d][j] = src.vel[d][i];
+        dst.off[d][j] = o[d];
+        dst.vel[d][j] = v[d];
     }
-    if (src.mass) dst.mass[j] = src.mass[i];
-    if (src.id) dst.id[j] = src.id[i];
+    if (src.mass) dst.mass[j] = m;
+    if (src.id) dst.id[j] = id;
 }
 
 // ---------------------------------------------------------------- items
@@ -584,19 +597,23 @@ template <> struct AccOps<unsigned long long> {
     }
 };
 
-template <int KIND, int COMP, typename ACC>
+template <int KIND, int COMP, typename ACC, bool ALIAS>
 __device__ __forceinline__ void deposit_component(const DepArgs &A, const ItemRange &r, int NC, const int *cst,
                                                   const float4 *sp, const float4 *sv, ACC *cube) {
     constexpr int MASK = Comp<KIND, COMP>::MASK;
     constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
     constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
     constexpr int NS = SX * SY * SZ;
+    static_assert(MAXB * MAXB * MAXB <= 2 * DEP_WG, "two cells per thread");
     const int B = A.g.B;
-    // B. per-cell slots
-    for (int c = threadIdx.x; c < NC; c += DEP_WG) {
-        ACC acc[NS];
+    // B. per-cell slots in registers: thread t takes cells t and t + DEP_WG
+    ACC acc[2][NS];
 #pragma unroll
-        for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
+    for (int h = 0; h < 2; ++h) {
+        const int c = threadIdx.x + h * DEP_WG;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[h][k] = ACC(0);
+        if (c >= NC) continue;
         const int j1 = cst[c + 1];
         for (int j = cst[c]; j < j1; ++j) {
             const float4 d = sp[j];
@@ -620,10 +637,19 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, const ItemRa
                 for (int ky = 0; ky < SY; ++ky)
 #pragma unroll
                     for (int kx = 0; kx < SX; ++kx)
-                        acc[kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
+                        acc[h][kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
         }
+    }
+    // the slots overwrite the sorted particles when they share LDS
+    if constexpr (ALIAS) __syncthreads();
+    // slot-major layout: consecutive cells in consecutive banks
 #pragma unroll
-        for (int k = 0; k < NS; ++k) cube[c * NS + k] = acc[k];
+    for (int h = 0; h < 2; ++h) {
+        const int c = threadIdx.x + h * DEP_WG;
+        if (c < NC) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) cube[k * NC + c] = acc[h][k];
+        }
     }
     __syncthreads();
     // C. node gather: nodes with local coordinate in [lo, B] per axis, lo = -1 on link axes
@@ -647,7 +673,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, const ItemRa
                     const int cx = ax - ox;
                     if (cx < 0 || cx >= B) continue;
                     const int slot = (ox - ((MASK & 1) ? -1 : 0)) + SX * ((oy - ((MASK & 2) ? -1 : 0)) + SY * (oz - ((MASK & 4) ? -1 : 0)));
-                    v += cube[(cx + B * (cy + B * cz)) * NS + slot];
+                    v += cube[slot * NC + (cx + B * (cy + B * cz))];
                 }
             }
         }
@@ -669,12 +695,20 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
     constexpr int SUBP = KindC<KIND>::SUBP;
     constexpr int PER = SUBP / DEP_WG;
     constexpr int SPLIT = ITEM_P / SUBP;
-    __shared__ int cnt[MAXB * MAXB * MAXB];
-    __shared__ int cst[MAXB * MAXB * MAXB + 1];
+    constexpr int NCMAX = MAXB * MAXB * MAXB;
+    // scalar: the per-cell slots reuse the sorted particles' LDS
+    constexpr bool ALIAS = KIND == 0;
+    constexpr size_t SP_B = sizeof(float4) * SUBP;
+    constexpr size_t SV_B = KIND > 0 ? sizeof(float4) * SUBP : 0;
+    constexpr size_t CUBE_B = sizeof(ACC) * NCMAX * KindC<KIND>::NSMAX;
+    constexpr size_t POOL_B = ALIAS ? (SP_B > CUBE_B ? SP_B : CUBE_B) : SP_B + SV_B + CUBE_B;
+    __shared__ int cnt[NCMAX];
+    __shared__ int cst[NCMAX + 1];
     __shared__ int scan_sh[DEP_WG];
-    __shared__ float4 sp[SUBP];
-    __shared__ float4 sv[KIND > 0 ? SUBP : 1];
-    __shared__ ACC cube[MAXB * MAXB * MAXB * KindC<KIND>::NSMAX];
+    __shared__ __align__(16) unsigned char pool[POOL_B];
+    float4 *sp = reinterpret_cast<float4 *>(pool);
+    float4 *sv = reinterpret_cast<float4 *>(pool + SP_B);
+    ACC *cube = reinterpret_cast<ACC *>(ALIAS ? pool : pool + SP_B + SV_B);
     ItemRange r;
     if (!item_range(blockIdx.x / SPLIT, A.start, A.istart, A.item_bucket, A.g, r)) return;
     const int p0 = r.p0 + (int)(blockIdx.x % SPLIT) * SUBP;
@@ -724,18 +758,18 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
     }
     __syncthreads();
     if constexpr (KIND == 0) {
-        deposit_component<0, 0, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<0, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
     } else if constexpr (KIND == 1) {
-        deposit_component<1, 0, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<1, 1, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<1, 2, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<1, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<1, 1, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<1, 2, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
     } else {
-        deposit_component<2, 0, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 1, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 2, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 3, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 4, ACC>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 5, ACC>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 1, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 2, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 3, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 4, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
+        deposit_component<2, 5, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
     }
 }
 
@@ -841,17 +875,54 @@ __device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, 
 
 static constexpr int GKD_DIRECT_MAX = 64;
 
+// histogram of the new buckets of one item's particles (bucket b): particles that stay in
+// the brick are counted with one LDS atomic per wave, moves to the 26 neighbouring bricks
+// in a 27-entry LDS table, farther moves straight in HBM; the table is flushed once per
+// item (gkd_hist_flush), so a dense brick's many items do not serialise on its counter
+__device__ __forceinline__ void gkd_hist_add(const GkdArgs &A, int b, int *h27, bool act, int newcell) {
+    const int nb = act ? bucket_of(newcell, A.g) : b;
+    const bool stay = act && nb == b;
+    const unsigned long long st = __ballot(stay);
+    if (lane_id() == 0 && st) atomicAdd(&h27[13], __popcll(st));
+    if (!act || stay) return;
+    const int NB = A.g.NB;
+    const int bx = nb % NB, by = (nb / NB) % NB, bz = nb / (NB * NB);
+    const int rx = b % NB, ry = (b / NB) % NB, rz = b / (NB * NB);
+    auto delta = [NB](int d) {
+        if (d == NB - 1 && d > 1) return -1;
+        if (d == 1 - NB && d < -1) return 1;
+        return d;
+    };
+    const int dx = delta(bx - rx), dy = delta(by - ry), dz = delta(bz - rz);
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
+        atomicAdd(&h27[13 + dx + 3 * dy + 9 * dz], 1);
+    else
+        atomicAdd(&A.next_counts[nb], 1);
+}
+
+// after a barrier: one HBM atomic per non-empty neighbour entry
+__device__ __forceinline__ void gkd_hist_flush(const GkdArgs &A, int b, const int *h27) {
+    const int t = threadIdx.x;
+    if (t >= 27 || h27[t] == 0) return;
+    const int NB = A.g.NB;
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    const int bx = (b % NB + dx + NB) % NB, by = ((b / NB) % NB + dy + NB) % NB, bz = (b / (NB * NB) + dz + NB) % NB;
+    atomicAdd(&A.next_counts[bx + NB * (by + NB * bz)], h27[t]);
+}
+
 __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     constexpr int PW = MAXB + 3;  // phi window [o-1, o+B+1]
     constexpr int FW = MAXB + 1;  // force nodes [o, o+B]
     __shared__ float phw[PW * PW * PW];
     __shared__ float frc[3][FW * FW * FW];
     __shared__ unsigned int red;
-    __shared__ BlockAgg<DEP_WG> agg;
+    __shared__ int h27[27];
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
     const int N = A.g.N, B = A.g.B;
     if (threadIdx.x == 0) red = 0u;
+    if (threadIdx.x < 27) h27[threadIdx.x] = 0;
+    __syncthreads();
     float vmax = 0.0f;
     if (!A.phi || r.p1 - r.p0 < GKD_DIRECT_MAX) {
         // sparse item (no window) or drift only (no potential: a = 0)
@@ -868,7 +939,7 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
                 if (A.phi) gather_direct(A, cx, cy, cz, u, a);
                 newcell = kick_drift_store(A, p, cx, cy, cz, u, vin, a, vmax);
             }
-            block_agg_inc<DEP_WG>(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act, agg);
+            gkd_hist_add(A, r.b, h27, act, newcell);
         }
     } else {
         const int pw = B + 3, fw = B + 1;
@@ -928,14 +999,13 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
                         }
                 newcell = kick_drift_store(A, p, cx, cy, cz, u, vin, a, vmax);
             }
-            block_agg_inc<DEP_WG>(A.next_counts, act ? bucket_of(newcell, A.g) : 0, act, agg);
+            gkd_hist_add(A, r.b, h27, act, newcell);
         }
     }
-    if (A.vmax2) {
-        atomicMax(&red, __float_as_uint(vmax));
-        __syncthreads();
-        if (threadIdx.x == 0) atomicMax(A.vmax2, red);
-    }
+    if (A.vmax2) atomicMax(&red, __float_as_uint(vmax));
+    __syncthreads();
+    gkd_hist_flush(A, r.b, h27);
+    if (A.vmax2 && threadIdx.x == 0) atomicMax(A.vmax2, red);
 }
 
 // ---------------------------------------------------------------- F64 plain path
@@ -1068,7 +1138,7 @@ __global__ void gkd_f64_kernel(Gkd64Args A) {
 
 // ---------------------------------------------------------------- download / counts
 __global__ void download_f32_kernel(int64_t np, Soa32 s, int N, double *__restrict__ pos, double *__restrict__ vel,
-                                    double *__restrict__ mass) {
+                                    double *__restrict__ mass, double mass_global) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int cell = s.cell[i];
@@ -1077,7 +1147,7 @@ __global__ void download_f32_kernel(int64_t np, Soa32 s, int N, double *__restri
         if (pos) pos[3 * i + d] = add_rn((double)c[d], (double)s.off[d][i]) / (double)N;
         if (vel) vel[3 * i + d] = (double)s.vel[d][i];
     }
-    if (mass) mass[i] = s.mass ? (double)s.mass[i] : 1.0;
+    if (mass) mass[i] = s.mass ? (double)s.mass[i] : mass_global;
 }
 
 __global__ void download_f64_kernel(int64_t np, Soa64 s, double *__restrict__ pos, double *__restrict__ vel) {
@@ -1400,7 +1470,7 @@ int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, 
     int rc = LF2MI_OK;
     if (p->dtype == LF2MI_F32) {
         const Soa32 &s = p->b32[p->cur];
-        download_f32_kernel<<<grid1(np), 256, 0, st>>>(np, s, p->N, dpos, dvel, dmass);
+        download_f32_kernel<<<grid1(np), 256, 0, st>>>(np, s, p->N, dpos, dvel, dmass, p->mass_global);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && pos) e = hipMemcpyAsync(pos, dpos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, dvel, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
