@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="start with a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -121,6 +123,12 @@ def main():
     from latfield2_amd import _lib
 
     torch.cuda.set_device(0)
+    if args.calib:
+        a = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     N = args.n
     Np = N ** 3
     dt = 0.01
@@ -153,6 +161,25 @@ def main():
         v["GBps"] = round(b / (v["avg_ms"] * 1e-3) / 1e9, 1) if b else None
         v["avg_ms"] = round(v["avg_ms"], 4)
         v["ms_per_step"] = round(v["ms_per_step"], 4)
+    # the clustered deposit timed both ways on the same particles: FP32 float atomics and
+    # int64 fixed point (2^-32 units, run-to-run reproducible, converted to FP32 after)
+    dep = {}
+    for name, acc in (("float", lf.ACC_NATIVE), ("fixed64", lf.ACC_FIXED64)):
+        rho.zero()
+        parts.deposit(lf.PROJ_SCALAR, rho, lf.DEP_PERIODIC, acc)
+        torch.cuda.synchronize()
+        _lib.profile_reset()
+        _lib.profile_enable(True)
+        for _ in range(5):
+            parts.deposit(lf.PROJ_SCALAR, rho, lf.DEP_PERIODIC, acc)
+        torch.cuda.synchronize()
+        _lib.profile_enable(False)
+        pr = _lib.profile_read()
+        ms_dep = sum(t / c for t, c in pr.values())
+        ab_dep = algorithmic_bytes("deposit_f32_scalar", N, Np, 4)
+        dep[name] = {"ms": round(ms_dep, 4), "GBps": round(ab_dep / (ms_dep * 1e-3) / 1e9, 1),
+                     "kernels": {k: round(t / c, 4) for k, (t, c) in pr.items()}}
+    log(f"clustered deposit: {dep}")
     phi32 = phi.interior()[..., 0].double().cpu()
     del parts, rho, phi, plan
     torch.cuda.empty_cache()
@@ -211,6 +238,7 @@ def main():
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "step_frac_hbm_peak": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
+        "clustered_deposit": dep,
         **extra,
     }
     print(json.dumps(out), flush=True)

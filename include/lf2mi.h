@@ -178,6 +178,15 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
 int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, int64_t *id,
                          int32_t *cell, void *stream);
 
+/* The particles whose cell lies in the box [lo[d], hi[d]) on every axis (0 <= lo < hi <= N,
+ * no wrap), in no particular order: positions (box units) and velocities np*3, masses np,
+ * host arrays of room max_out.  *n_out receives the count; LF2MI_ERANGE when it exceeds
+ * max_out.  No counterpart in the reference; it serves subvolume checks of meshes whose
+ * particles do not fit the host at once (Particles::getParticles over a region). */
+int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_t max_out,
+                             double *pos, double *vel, double *mass, int64_t *n_out,
+                             void *stream);
+
 /* per-cell particle counts into a device int32 array of N^3 entries (bit-exact with
  * the histogram of Particles::getPartCoord keys) */
 int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream);

@@ -258,6 +258,19 @@ class Particles:
             out["id"] = ids
         return out
 
+    def download_box(self, lo, hi, max_out: int):
+        """particles whose cell lies in [lo, hi) per axis: dict of pos, vel, mass (numpy)"""
+        lo_a = (ctypes.c_int32 * 3)(*[int(v) for v in lo])
+        hi_a = (ctypes.c_int32 * 3)(*[int(v) for v in hi])
+        pos = np.zeros((max_out, 3))
+        vel = np.zeros((max_out, 3))
+        mass = np.zeros(max_out)
+        n = ctypes.c_int64(0)
+        call("lf2mi_parts_download_box", self.h, lo_a, hi_a, int(max_out), _dp(pos), _dp(vel), _dp(mass),
+             ctypes.byref(n), _stream())
+        k = n.value
+        return {"pos": pos[:k], "vel": vel[:k], "mass": mass[:k]}
+
     def cell_counts(self):
         torch = _torch()
         c = torch.empty(self.N ** 3, dtype=torch.int32, device="cuda")

@@ -60,6 +60,7 @@ SIGNATURES = {
     "lf2mi_parts_upload": [_vp, _i64, _dp, _dp, _dp, _i64p, _i64p, _vp],
     "lf2mi_parts_download": [_vp, _dp, _dp, _dp, _i64p, _i32p, _vp],
     "lf2mi_parts_cell_counts": [_vp, _vp, _vp],
+    "lf2mi_parts_download_box": [_vp, _i32p, _i32p, _i64, _dp, _dp, _dp, _i64p, _vp],
     "lf2mi_parts_ic_lattice": [_vp, _i, _d, _d, _u64, _vp],
     "lf2mi_parts_ic_uniform": [_vp, _i64, _d, _u64, _vp],
     "lf2mi_parts_ic_blobs": [_vp, _i64, _i, _d, _d, _d, _d, _u64, _vp],

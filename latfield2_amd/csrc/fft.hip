@@ -363,12 +363,18 @@ template <typename T> struct Tune;
 template <> struct Tune<float> {
     static constexpr int WG = 256;
     template <int N> static constexpr int strided_L() { return N >= 1024 ? 8 : 16; }
-    template <int N> static constexpr int x_L() { return N >= 512 ? 6 : (N >= 128 ? 12 : 24); }
+    template <int N> static constexpr int green_L() { return N >= 512 ? 8 : 16; }
+    // x pass: lines per tile for one component (row pairs), and the widest tile, used when
+    // the field has more components than the narrow tile holds
+    template <int N> static constexpr int x_L() { return N >= 512 ? 4 : (N >= 128 ? 12 : 24); }
+    template <int N> static constexpr int x_L_max() { return N >= 512 ? 6 : (N >= 128 ? 12 : 24); }
 };
 template <> struct Tune<double> {
     static constexpr int WG = 256;
     template <int N> static constexpr int strided_L() { return N >= 1024 ? 4 : 8; }
+    template <int N> static constexpr int green_L() { return N >= 1024 ? 4 : 8; }
     template <int N> static constexpr int x_L() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
+    template <int N> static constexpr int x_L_max() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
 };
 
 }  // namespace lf2mi
@@ -391,24 +397,26 @@ template <typename T, int N>
 static int launch_strided(const StridedArgs<T> &A, int S, int mode, int nbatch, hipStream_t st) {
     constexpr int WG = Tune<T>::WG;
     constexpr int L = Tune<T>::template strided_L<N>();
-    const size_t shm = sizeof(cplx<T>) * N * L;
-    dim3 grid((A.ncols + L - 1) / L, nbatch);
+    constexpr int LG = Tune<T>::template green_L<N>();
     LF2MI_PROF(mode == 1 ? "fft_y_green" : (S < 0 ? "fft_strided_fwd" : "fft_strided_inv"), st);
-    if (mode == 1)
-        fft_strided_kernel<T, N, L, WG, -1, 1><<<grid, WG, shm, st>>>(A);
-    else if (S < 0)
-        fft_strided_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
-    else
-        fft_strided_kernel<T, N, L, WG, +1, 0><<<grid, WG, shm, st>>>(A);
+    if (mode == 1) {
+        dim3 grid((A.ncols + LG - 1) / LG, nbatch);
+        fft_strided_kernel<T, N, LG, WG, -1, 1><<<grid, WG, sizeof(cplx<T>) * N * LG, st>>>(A);
+    } else {
+        dim3 grid((A.ncols + L - 1) / L, nbatch);
+        const size_t shm = sizeof(cplx<T>) * N * L;
+        if (S < 0)
+            fft_strided_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
+        else
+            fft_strided_kernel<T, N, L, WG, +1, 0><<<grid, WG, shm, st>>>(A);
+    }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
 
-template <typename T, int N>
-static int launch_x(const XArgs<T> &A, int S, hipStream_t st) {
+template <typename T, int N, int L>
+static int launch_x_tile(const XArgs<T> &A, int S, hipStream_t st) {
     constexpr int WG = Tune<T>::WG;
-    constexpr int L = Tune<T>::template x_L<N>();
-    if (A.C > L) return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", A.C, L);
     const int P = L / A.C;
     const int64_t npairs = (int64_t)N * N / 2;
     const unsigned grid = (unsigned)((npairs + P - 1) / P);
@@ -420,6 +428,17 @@ static int launch_x(const XArgs<T> &A, int S, hipStream_t st) {
         fft_x_c2r_kernel<T, N, L, WG><<<grid, WG, shm, st>>>(A);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
+}
+
+template <typename T, int N>
+static int launch_x(const XArgs<T> &A, int S, hipStream_t st) {
+    constexpr int L = Tune<T>::template x_L<N>();
+    constexpr int LMAX = Tune<T>::template x_L_max<N>();
+    if (A.C <= L) return launch_x_tile<T, N, L>(A, S, st);
+    if constexpr (LMAX > L) {
+        if (A.C <= LMAX) return launch_x_tile<T, N, LMAX>(A, S, st);
+    }
+    return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", A.C, LMAX);
 }
 
 #define LF2MI_DISPATCH_N(N_, CALL)                              \

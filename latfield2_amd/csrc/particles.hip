@@ -1159,6 +1159,55 @@ __global__ void download_f64_kernel(int64_t np, Soa64 s, double *__restrict__ po
     }
 }
 
+// particles whose cell lies in the box [lo, hi) per axis, compacted through a cursor
+struct BoxArgs {
+    int lo[3], hi[3];
+    int N;
+    int64_t np, max_out;
+    int *cursor;
+    double *pos, *vel, *mass;
+    double mass_global;
+};
+__global__ void download_box_f32_kernel(BoxArgs A, Soa32 s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    int c[3] = {0, 0, 0};
+    if (i < A.np) {
+        const int cell = s.cell[i];
+        if (cell >= 0) {
+            c[0] = cell % A.N;
+            c[1] = (cell / A.N) % A.N;
+            c[2] = cell / (A.N * A.N);
+            in = true;
+            for (int d = 0; d < 3; ++d) in = in && c[d] >= A.lo[d] && c[d] < A.hi[d];
+        }
+    }
+    const int j = agg_inc(A.cursor, 0, in);
+    if (!in || j >= A.max_out) return;
+    for (int d = 0; d < 3; ++d) {
+        A.pos[3 * (int64_t)j + d] = add_rn((double)c[d], (double)s.off[d][i]) / (double)A.N;
+        A.vel[3 * (int64_t)j + d] = (double)s.vel[d][i];
+    }
+    A.mass[j] = s.mass ? (double)s.mass[i] : A.mass_global;
+}
+__global__ void download_box_f64_kernel(BoxArgs A, Soa64 s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (i < A.np) {
+        const int cell = s.cell[i];
+        const int c[3] = {cell % A.N, (cell / A.N) % A.N, cell / (A.N * A.N)};
+        in = true;
+        for (int d = 0; d < 3; ++d) in = in && c[d] >= A.lo[d] && c[d] < A.hi[d];
+    }
+    const int j = agg_inc(A.cursor, 0, in);
+    if (!in || j >= A.max_out) return;
+    for (int d = 0; d < 3; ++d) {
+        A.pos[3 * (int64_t)j + d] = s.pos[d][i];
+        A.vel[3 * (int64_t)j + d] = s.vel[d][i];
+    }
+    A.mass[j] = s.mass ? s.mass[i] : A.mass_global;
+}
+
 __global__ void cell_count_kernel(int64_t np, const int32_t *__restrict__ cell, int32_t *__restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < np && cell[i] >= 0) atomicAdd(&counts[cell[i]], 1);
@@ -1503,6 +1552,57 @@ int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, 
     hipFree(dvel);
     hipFree(dmass);
     return rc;
+}
+
+int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_t max_out, double *pos, double *vel,
+                             double *mass, int64_t *n_out, void *stream) {
+    if (!p || !lo || !hi || !n_out || max_out < 0 || (max_out > 0 && (!pos || !vel || !mass)))
+        return fail(LF2MI_EINVAL, "lf2mi_parts_download_box: bad argument");
+    BoxArgs A{};
+    for (int d = 0; d < 3; ++d) {
+        if (lo[d] < 0 || hi[d] > p->N || lo[d] >= hi[d]) return fail(LF2MI_ERANGE, "box [%d, %d) outside [0, %d)", lo[d], hi[d], p->N);
+        A.lo[d] = lo[d];
+        A.hi[d] = hi[d];
+    }
+    *n_out = 0;
+    if (p->np == 0) return LF2MI_OK;
+    hipStream_t st = as_stream(stream);
+    const int64_t m = std::max<int64_t>(max_out, 1);
+    double *d_pos = nullptr, *d_vel = nullptr, *d_mass = nullptr;
+    int *d_cur = nullptr;
+    hipError_t e = hipMalloc((void **)&d_pos, m * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_vel, m * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_mass, m * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_cur, sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(d_cur, 0, sizeof(int), st);
+    int cnt = 0;
+    if (e == hipSuccess) {
+        A.N = p->N;
+        A.np = p->np;
+        A.max_out = max_out;
+        A.cursor = d_cur;
+        A.pos = d_pos;
+        A.vel = d_vel;
+        A.mass = d_mass;
+        A.mass_global = p->mass_global;
+        if (p->dtype == LF2MI_F32) download_box_f32_kernel<<<grid1(p->np), 256, 0, st>>>(A, p->b32[p->cur]);
+        else download_box_f64_kernel<<<grid1(p->np), 256, 0, st>>>(A, p->b64);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_cur, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const int64_t n = std::min<int64_t>(cnt, max_out);
+    if (e == hipSuccess && n > 0) e = hipMemcpy(pos, d_pos, n * 3 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && n > 0) e = hipMemcpy(vel, d_vel, n * 3 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && n > 0) e = hipMemcpy(mass, d_mass, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d_pos);
+    hipFree(d_vel);
+    hipFree(d_mass);
+    hipFree(d_cur);
+    if (e != hipSuccess) return hip_fail(e, "lf2mi_parts_download_box", __FILE__, __LINE__);
+    *n_out = cnt;
+    if (cnt > max_out) return fail(LF2MI_ERANGE, "%d particles in the box exceed max_out %lld", cnt, (long long)max_out);
+    return LF2MI_OK;
 }
 
 int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream) {

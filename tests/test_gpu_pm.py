@@ -38,6 +38,9 @@ def _run_step(N, prec, ic, dt_kick, dt_drift, seed, mass):
     else:
         parts.ic_blobs(np_, ic["nblobs"], 1.5, 2.0, 0.2, ic.get("sigma_v", 0.0), seed)
     d0 = _by_id(parts.download())
+    # binning of the initial state: cell keys and per-cell counts bit-exact
+    assert np.array_equal(d0["cell"], orc.bin_keys(d0["pos"], N))
+    assert np.array_equal(parts.cell_counts().cpu().numpy(), orc.cell_counts(d0["pos"], N))
     lat = lf.Lattice(3, N, 1)
     rho = lf.Field(lat, 1, DT[prec])
     phi = lf.Field(lat, 1, DT[prec])
@@ -82,11 +85,12 @@ def test_pm_step_c1_uniform_forces(prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
-def test_pm_step_lattice_gaussian_64(prec, tol):
-    """the kick-drift configuration at 64^3: grid + N(0, 0.3 cells), v ~ N(0, 0.01), dt 0.01,
-    mass 1/Np (mean density 1)"""
-    N, dt = 64, 0.01
-    announce(f"pm_step lattice 64 {prec}")
+@pytest.mark.parametrize("N", [64, 256])
+def test_pm_step_lattice_gaussian(N, prec, tol):
+    """the kick-drift configuration (256^3 mesh with 256^3 particles is its full size): grid +
+    N(0, 0.3 cells), v ~ N(0, 0.01), dt 0.01, mass 1/Np (mean density 1)"""
+    dt = 0.01
+    announce(f"pm_step lattice {N} {prec}")
     ic = {"kind": "lattice", "ng": N, "sigma_cells": 0.3, "sigma_v": 0.01}
     d0, d1, rho, phi = _run_step(N, prec, ic, dt, dt, seed=2, mass=1.0 / N ** 3)
     _check_step(N, prec, d0, d1, rho, phi, dt, dt, 1.0 / N ** 3, tol)
@@ -100,3 +104,57 @@ def test_pm_step_blobs_64(prec, tol):
     ic = {"kind": "blobs", "np": N ** 3, "nblobs": 512, "sigma_v": 0.01}
     d0, d1, rho, phi = _run_step(N, prec, ic, dt, dt, seed=3, mass=1.0 / N ** 3)
     _check_step(N, prec, d0, d1, rho, phi, dt, dt, 1.0 / N ** 3, tol)
+
+
+def test_clustered_deposit_512_float_and_fixed_point():
+    """512^3 mesh, 512^3 particles in 4096 Pareto(1.5) blobs (sigma 2 cells, 20% uniform):
+    the FP32 deposit with float atomics and with int64 fixed point (2^-32) against the FP64
+    oracle deposit of the same positions; the fixed-point deposit is bitwise reproducible;
+    then a kick-only PM step from the fixed-point deposit against the oracle step"""
+    N = 512
+    np_ = N ** 3
+    mass = 1.0 / np_
+    announce("clustered deposit 512^3, float and int64 fixed point")
+    parts = lf.Particles(N, np_, lf.F32, ids=True, mass=mass)
+    parts.ic_blobs(np_, 4096, 1.5, 2.0, 0.2, 0.01, 3)
+    lat = lf.Lattice(3, N, 1)
+    s = slice(1, N + 1)
+    got = {}
+    first_fixed = None
+    for name, acc in (("native", lf.ACC_NATIVE), ("fixed", lf.ACC_FIXED64), ("fixed again", lf.ACC_FIXED64)):
+        f = lf.Field(lat, 1, lf.F32)
+        f.zero()
+        parts.deposit(lf.PROJ_SCALAR, f, lf.DEP_PERIODIC, acc)
+        torch.cuda.synchronize()
+        if name == "fixed again":
+            assert torch.equal(f.t, first_fixed)  # run-to-run reproducible
+        else:
+            got[name] = f.t[s, s, s, 0].double().cpu().numpy()
+            if name == "fixed":
+                first_fixed = f.t.clone()
+        del f
+    del first_fixed
+    d0 = _by_id(parts.download())
+    orc.set_threads(16)
+    ref = orc.real_field(N, 1, 1)
+    orc.scalar_project(d0["pos"], ref, N, 1, mass_global=mass)
+    orc.scalar_comm(ref, N, 1)
+    errs = {k: rel_l2(v, ref[s, s, s, 0]) for k, v in got.items()}
+    print(f"deposit rel-L2 against the FP64 oracle: {errs}")
+    assert errs["native"] <= 1e-6 and errs["fixed"] <= 1e-6
+    del ref, got
+    # kick-only step (no drift) from the fixed-point deposit
+    rho = lf.Field(lat, 1, lf.F32)
+    phi = lf.Field(lat, 1, lf.F32)
+    plan = lf.PlanFFT(rho, None)
+    lf.pm_step(parts, plan, rho, phi, 0.01, 0.0, acc=lf.ACC_FIXED64)
+    torch.cuda.synchronize()
+    d1 = _by_id(parts.download())
+    pos = np.ascontiguousarray(d0["pos"].copy())
+    vel = np.ascontiguousarray(d0["vel"].copy())
+    out = orc.pm_step(pos, vel, N, 0.01, 0.0, mass_global=mass, want_acc=False)
+    e_phi = rel_l2(real_np(phi)[s, s, s, 0], out["phi"][s, s, s, 0])
+    e_dv = rel_l2(d1["vel"] - d0["vel"], vel - d0["vel"])
+    print(f"phi rel-L2 {e_phi:.2e}, velocity-change rel-L2 {e_dv:.2e}")
+    assert np.array_equal(d1["cell"], d0["cell"])
+    assert e_phi <= 1e-5 and e_dv <= 1e-5
