@@ -124,11 +124,15 @@ def main():
 
     torch.cuda.set_device(0)
     if args.calib:
+        # 1 GiB read + 1 GiB write of a plain copy, its source pushed out of the on-chip and
+        # memory-side caches by 4 GiB of other writes first
         a = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        flush = torch.zeros(2 ** 30, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
+        torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
-        del a, b
+        del a, b, flush
     N = args.n
     Np = N ** 3
     dt = 0.01

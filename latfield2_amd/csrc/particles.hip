@@ -873,7 +873,10 @@ __device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, 
             }
 }
 
-static constexpr int GKD_DIRECT_MAX = 64;
+// items with fewer particles gather phi straight from HBM (through L1) instead of staging
+// the brick's window and force table in LDS, whose fixed cost only pays off for fuller
+// bricks (measured at 512^3 on clustered particles: 192 beats 64 and 512)
+static constexpr int GKD_DIRECT_MAX = 192;
 
 // histogram of the new buckets of one item's particles (bucket b): particles that stay in
 // the brick are counted with one LDS atomic per wave, moves to the 26 neighbouring bricks
