@@ -522,20 +522,27 @@ __device__ __forceinline__ bool item_range(int item, const int *__restrict__ sta
 // The reference accumulates, per cell, the contributions of the cell's particles to the
 // nodes around it (localCube[8] / vi[36] / tii[24] + tij[54], particles/projections.hpp:
 // 140-179, :1034-1122, :1421-1521) and then adds them to the field.  Here one workgroup
-// takes an item (a chunk of one brick's particles) and
+// takes an item (a chunk of at most ITEM_P particles of one brick) and
 //   A. counting-sorts the chunk by cell in LDS (one int LDS atomic per particle);
-//   B. one thread per cell accumulates its particles into per-cell slots in registers,
-//      one component at a time, in a fixed order, and stores them to LDS;
-//   C. one thread per node sums the slots of the cells around it (fixed order) and adds
-//      the node value into the field with one global atomic (periodic wrap or the
-//      unfolded ghost layer).
-// Slots of a component: per axis the node offsets a cell's particles reach, {0, 1} on the
-// vertex grid and {-1, 0, +1} on the link grid (nodes at n + 1/2), so 8 (scalar, T_ii),
-// 12 (vector component) or 18 (T_ij) slots.
+//   B. each lane takes a run of DEP_PER consecutive sorted particles and sums their node
+//      terms per cell in registers; a run's inner cells go to the LDS node window with LDS
+//      atomics, its last cell is combined across the wave with a segmented DPP sum (equal
+//      cells are adjacent lanes after the sort) and added once per run of lanes, so a
+//      crowded cell costs the same as a sparse one.  The window holds 64-bit fixed point
+//      (integer LDS atomics: the FP32 LDS atomic add is many times slower on gfx950): per
+//      item and component a power-of-two scale puts the bound sum_p |q_p| (from the item's
+//      maxima of |q| and |v|) just below 2^61, so the window sums are exact and the only
+//      rounding past the FP32 run sums is the quantum 2^-61 of that bound;
+//   C. the node window (all components, component-fastest) is added into the field with
+//      one global atomic per non-zero value (periodic wrap or the unfolded ghost layer).
+// Node offsets per axis: {0, 1} on the vertex grid and {-1, 0, +1} on the link grid
+// (nodes at n + 1/2), so 8 (scalar, T_ii), 12 (vector component) or 18 (T_ij) terms per
+// particle and component; the window spans nodes [LO, B] per axis, LO = -1 when a
+// component of the kind sits on links.
 template <int KIND> struct KindC;
-template <> struct KindC<0> { static constexpr int C = 1; static constexpr int SUBP = ITEM_P; static constexpr int NSMAX = 8; };
-template <> struct KindC<1> { static constexpr int C = 3; static constexpr int SUBP = ITEM_P / 2; static constexpr int NSMAX = 12; };
-template <> struct KindC<2> { static constexpr int C = 6; static constexpr int SUBP = ITEM_P / 2; static constexpr int NSMAX = 18; };
+template <> struct KindC<0> { static constexpr int C = 1; static constexpr int LO = 0; };
+template <> struct KindC<1> { static constexpr int C = 3; static constexpr int LO = -1; };
+template <> struct KindC<2> { static constexpr int C = 6; static constexpr int LO = -1; };
 
 // link-grid weights of one axis as a 3-vector over node offsets {-1, 0, +1}
 // (projections.hpp:1058-1071: the branch at the link midpoint)
@@ -597,179 +604,316 @@ template <> struct AccOps<unsigned long long> {
     }
 };
 
-template <int KIND, int COMP, typename ACC, bool ALIAS>
-__device__ __forceinline__ void deposit_component(const DepArgs &A, const ItemRange &r, int NC, const int *cst,
-                                                  const float4 *sp, const float4 *sv, ACC *cube) {
+// ---- segmented sum over the lanes of a wave (DPP)
+// update_dpp with bound_ctrl off: lanes whose source is outside the row, or whose row the
+// row mask leaves out, get `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i32(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_val(float v) {
+    return __int_as_float(dpp_i32<CTRL, ROWMASK>(0, __float_as_int(v)));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned long long dpp_val(unsigned long long v) {
+    const unsigned lo = (unsigned)dpp_i32<CTRL, ROWMASK>(0, (int)(unsigned)v);
+    const unsigned hi = (unsigned)dpp_i32<CTRL, ROWMASK>(0, (int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <int CTRL, int ROWMASK, typename T, int NS>
+__device__ __forceinline__ void seg_step(int key, T (&v)[NS]) {
+    const bool same = dpp_i32<CTRL, ROWMASK>(-2, key) == key;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const T s = dpp_val<CTRL, ROWMASK>(v[k]);
+        v[k] += same ? s : T(0);
+    }
+}
+// Inclusive segmented scan over the 64 lanes (all active): lanes with equal keys must be
+// adjacent (keys >= 0, the sentinel -2 never matches); afterwards the last lane of each run
+// of equal keys holds the run's sum.  Within rows of 16: shifts by 1, 2, 4, 8; then lane 15
+// of rows 0 and 2 into rows 1 and 3, and lane 31 into rows 2 and 3.
+template <typename T, int NS>
+__device__ __forceinline__ void seg_scan(int key, T (&v)[NS]) {
+    seg_step<0x111, 0xf>(key, v);  // row_shr:1
+    seg_step<0x112, 0xf>(key, v);  // row_shr:2
+    seg_step<0x114, 0xf>(key, v);  // row_shr:4
+    seg_step<0x118, 0xf>(key, v);  // row_shr:8
+    seg_step<0x142, 0xa>(key, v);  // row_bcast:15
+    seg_step<0x143, 0xc>(key, v);  // row_bcast:31
+}
+
+static constexpr int DEP_PER = ITEM_P / DEP_WG;  // sorted particles per lane run
+
+// window value of a run sum: FP32 sums in the item's fixed point, fixed-point terms as they are
+__device__ __forceinline__ unsigned long long to_window(float x, double scale) {
+    return (unsigned long long)llrint((double)x * scale);
+}
+__device__ __forceinline__ unsigned long long to_window(unsigned long long x, double) { return x; }
+
+// add a cell's NS node terms into the LDS node window of one component (edge NWE,
+// origin LO): node offsets per axis start at -1 on link axes, 0 on vertex axes
+template <int MASK, int LO, typename ACC, int NS>
+__device__ __forceinline__ void window_add(unsigned long long *nd, int cell, int B, int NWE, const ACC (&acc)[NS],
+                                           double scale) {
+    constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
+    const int lx = cell % B, ly = (cell / B) % B, lz = cell / (B * B);
+    const int base = (lx + ((MASK & 1) ? -1 : 0) - LO) + NWE * ((ly + ((MASK & 2) ? -1 : 0) - LO) +
+                                                              NWE * (lz + ((MASK & 4) ? -1 : 0) - LO));
+#pragma unroll
+    for (int kz = 0; kz < SZ; ++kz)
+#pragma unroll
+        for (int ky = 0; ky < SY; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < SX; ++kx)
+                atomicAdd(&nd[base + kx + NWE * (ky + NWE * kz)], to_window(acc[kx + SX * (ky + SY * kz)], scale));
+}
+
+// phase B for one component: the lane's run of sorted particles (cells key[k], -1 past the
+// item's end), its last cell carried across the wave
+template <int KIND, int COMP, typename ACC, int BC>
+__device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int NWE, const int (&key)[DEP_PER],
+                                                  const float4 (&pd)[DEP_PER], const float4 (&pv)[DEP_PER],
+                                                  int ckey, bool any_shared, bool carry_end, unsigned long long *nodes,
+                                                  double scale) {
     constexpr int MASK = Comp<KIND, COMP>::MASK;
     constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
     constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
     constexpr int NS = SX * SY * SZ;
-    static_assert(MAXB * MAXB * MAXB <= 2 * DEP_WG, "two cells per thread");
-    const int B = A.g.B;
-    // B. per-cell slots in registers: thread t takes cells t and t + DEP_WG
-    ACC acc[2][NS];
+    constexpr int LO = KindC<KIND>::LO;
+    constexpr int NWMAX = MAXB + 1 - LO;
+    unsigned long long *nd = nodes + COMP * NWMAX * NWMAX * NWMAX;
+    ACC acc[NS];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int c = threadIdx.x + h * DEP_WG;
+    for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
+    int cur = -1;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) acc[h][k] = ACC(0);
-        if (c >= NC) continue;
-        const int j1 = cst[c + 1];
-        for (int j = cst[c]; j < j1; ++j) {
-            const float4 d = sp[j];
-            float q = d.w;
-            if constexpr (VA >= 0) {
-                const float4 v = sv[j];
-                const float va = VA == 0 ? v.x : (VA == 1 ? v.y : v.z);
-                q *= va;
-                if constexpr (VB >= 0) q *= (VB == 0 ? v.x : (VB == 1 ? v.y : v.z));
+    for (int p = 0; p < DEP_PER; ++p) {
+        const int c = key[p];
+        if (c < 0) continue;  // past the end: every later particle of the run is too
+        if (c != cur) {
+            if (cur >= 0) {
+                window_add<MASK, LO>(nd, cur, B, NWE, acc, scale);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
             }
-            float wx[3], wy[3], wz[3];
-            if constexpr (MASK & 1) link3(d.x, wx[0], wx[1], wx[2]);
-            else { wx[0] = 1.0f - d.x; wx[1] = d.x; }
-            if constexpr (MASK & 2) link3(d.y, wy[0], wy[1], wy[2]);
-            else { wy[0] = 1.0f - d.y; wy[1] = d.y; }
-            if constexpr (MASK & 4) link3(d.z, wz[0], wz[1], wz[2]);
-            else { wz[0] = 1.0f - d.z; wz[1] = d.z; }
-#pragma unroll
-            for (int kz = 0; kz < SZ; ++kz)
-#pragma unroll
-                for (int ky = 0; ky < SY; ++ky)
-#pragma unroll
-                    for (int kx = 0; kx < SX; ++kx)
-                        acc[h][kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
+            cur = c;
         }
-    }
-    // the slots overwrite the sorted particles when they share LDS
-    if constexpr (ALIAS) __syncthreads();
-    // slot-major layout: consecutive cells in consecutive banks
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int c = threadIdx.x + h * DEP_WG;
-        if (c < NC) {
-#pragma unroll
-            for (int k = 0; k < NS; ++k) cube[k * NC + c] = acc[h][k];
+        const float4 d = pd[p];
+        float q = d.w;
+        if constexpr (VA >= 0) {
+            const float4 v = pv[p];
+            q *= VA == 0 ? v.x : (VA == 1 ? v.y : v.z);
+            if constexpr (VB >= 0) q *= (VB == 0 ? v.x : (VB == 1 ? v.y : v.z));
         }
+        float wx[3], wy[3], wz[3];
+        if constexpr (MASK & 1) link3(d.x, wx[0], wx[1], wx[2]);
+        else { wx[0] = 1.0f - d.x; wx[1] = d.x; }
+        if constexpr (MASK & 2) link3(d.y, wy[0], wy[1], wy[2]);
+        else { wy[0] = 1.0f - d.y; wy[1] = d.y; }
+        if constexpr (MASK & 4) link3(d.z, wz[0], wz[1], wz[2]);
+        else { wz[0] = 1.0f - d.z; wz[1] = d.z; }
+#pragma unroll
+        for (int kz = 0; kz < SZ; ++kz)
+#pragma unroll
+            for (int ky = 0; ky < SY; ++ky)
+#pragma unroll
+                for (int kx = 0; kx < SX; ++kx)
+                    acc[kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
     }
+    // the run's last cell: summed over the lanes that share it (uniform branch)
+    if (any_shared) seg_scan(ckey, acc);
+    if (carry_end) window_add<MASK, LO>(nd, cur, B, NWE, acc, scale);
+}
+
+// item maxima: |q| and |v_d| as the bits of non-negative floats (ordered like the values)
+struct DepMax {
+    unsigned int q, v[3];
+};
+
+// per-item fixed-point scale of one component: 2^(61 - e) with sum_p |q_p v_a v_b| < 2^e
+template <int KIND, int COMP>
+__device__ __forceinline__ double window_scale(int n, const DepMax &m) {
+    constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
+    double bound = (double)n * (double)__uint_as_float(m.q);
+    if constexpr (VA >= 0) bound *= (double)__uint_as_float(m.v[VA]);
+    if constexpr (VB >= 0) bound *= (double)__uint_as_float(m.v[VB]);
+    if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;  // all zero (or not finite)
+    int e;
+    frexp(bound, &e);
+    return ldexp(1.0, 61 - e);
+}
+
+// grid: one workgroup per item
+template <int KIND, typename ACC, int BC>
+__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
+    constexpr int C = KindC<KIND>::C;
+    constexpr int LO = KindC<KIND>::LO;
+    constexpr int NWMAX = MAXB + 1 - LO;
+    constexpr int NW3 = NWMAX * NWMAX * NWMAX;
+    constexpr int NCMAX = MAXB * MAXB * MAXB;
+    constexpr bool FX = sizeof(ACC) == 8;  // global fixed point: terms quantised per particle
+    static_assert(2 * DEP_WG >= NCMAX, "two cells per thread in the scan");
+    // the sorted particles, then (aliased) the node window
+    constexpr size_t SP_B = sizeof(float4) * ITEM_P;
+    constexpr size_t SV_B = KIND > 0 ? sizeof(float4) * ITEM_P : 0;
+    constexpr size_t SK_B = sizeof(int) * ITEM_P;
+    constexpr size_t ND_B = sizeof(unsigned long long) * C * NW3;
+    constexpr size_t POOL_B = SP_B + SV_B + SK_B > ND_B ? SP_B + SV_B + SK_B : ND_B;
+    __shared__ int cnt[NCMAX];
+    __shared__ int cst[NCMAX + 1];
+    __shared__ int scan_sh[DEP_WG];
+    __shared__ DepMax smax;
+    __shared__ __align__(16) unsigned char pool[POOL_B];
+    float4 *sp = reinterpret_cast<float4 *>(pool);
+    float4 *sv = reinterpret_cast<float4 *>(pool + SP_B);
+    int *skey = reinterpret_cast<int *>(pool + SP_B + SV_B);
+    unsigned long long *nodes = reinterpret_cast<unsigned long long *>(pool);
+    ItemRange r;
+    if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
+    const int p0 = r.p0, n = r.p1 - r.p0;
+    const int B = BC ? BC : A.g.B, N = A.g.N;
+    const int NC = B * B * B, NWE = B + 1 - LO;
+    const int lane = lane_id();
+    for (int i = threadIdx.x; i < NC; i += DEP_WG) cnt[i] = 0;
+    if (threadIdx.x == 0) smax = DepMax{0u, {0u, 0u, 0u}};
     __syncthreads();
-    // C. node gather: nodes with local coordinate in [lo, B] per axis, lo = -1 on link axes
-    const int lox = (MASK & 1) ? -1 : 0, loy = (MASK & 2) ? -1 : 0, loz = (MASK & 4) ? -1 : 0;
-    const int nx = B + 1 - lox, ny = B + 1 - loy, nz = B + 1 - loz;
-    const int N = A.g.N;
-    for (int i = threadIdx.x; i < nx * ny * nz; i += DEP_WG) {
-        const int ax = i % nx + lox, ay = (i / nx) % ny + loy, az = i / (nx * ny) + loz;
-        ACC v = ACC(0);
-        // slot offset o = node - cell in [lo_axis, 1]; cell = node - o must lie in [0, B)
+    // A. load and rank by cell; item maxima of |q| and |v|
+    {
+        int lc[DEP_PER], rk[DEP_PER];
+        float4 qd[DEP_PER], qv[DEP_PER];
+        float mq = 0.0f, mv[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int oz = (MASK & 4) ? -1 : 0; oz <= 1; ++oz) {
-            const int cz = az - oz;
-            if (cz < 0 || cz >= B) continue;
+        for (int k = 0; k < DEP_PER; ++k) {
+            const int i = threadIdx.x + k * DEP_WG;
+            lc[k] = -1;
+            if (i < n) {
+                const int p = p0 + i;
+                const int cell = A.s.cell[p];
+                int lx, ly, lz;
+                decode_cell(cell, A.g, lx, ly, lz);
+                lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
+                qd[k] = make_float4(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p],
+                                    A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv);
+                mq = fmaxf(mq, fabsf(qd[k].w));
+                if constexpr (KIND > 0) {
+                    qv[k] = make_float4(A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p], 0.0f);
+                    mv[0] = fmaxf(mv[0], fabsf(qv[k].x));
+                    mv[1] = fmaxf(mv[1], fabsf(qv[k].y));
+                    mv[2] = fmaxf(mv[2], fabsf(qv[k].z));
+                }
+                rk[k] = atomicAdd(&cnt[lc[k]], 1);
+            }
+        }
+        if constexpr (!FX) {
 #pragma unroll
-            for (int oy = (MASK & 2) ? -1 : 0; oy <= 1; ++oy) {
-                const int cy = ay - oy;
-                if (cy < 0 || cy >= B) continue;
-#pragma unroll
-                for (int ox = (MASK & 1) ? -1 : 0; ox <= 1; ++ox) {
-                    const int cx = ax - ox;
-                    if (cx < 0 || cx >= B) continue;
-                    const int slot = (ox - ((MASK & 1) ? -1 : 0)) + SX * ((oy - ((MASK & 2) ? -1 : 0)) + SY * (oz - ((MASK & 4) ? -1 : 0)));
-                    v += cube[slot * NC + (cx + B * (cy + B * cz))];
+            for (int o = 32; o > 0; o >>= 1) {
+                mq = fmaxf(mq, __shfl_xor(mq, o));
+                if constexpr (KIND > 0) {
+                    mv[0] = fmaxf(mv[0], __shfl_xor(mv[0], o));
+                    mv[1] = fmaxf(mv[1], __shfl_xor(mv[1], o));
+                    mv[2] = fmaxf(mv[2], __shfl_xor(mv[2], o));
+                }
+            }
+            if (lane == 0) {
+                atomicMax(&smax.q, __float_as_uint(mq));
+                if constexpr (KIND > 0) {
+                    atomicMax(&smax.v[0], __float_as_uint(mv[0]));
+                    atomicMax(&smax.v[1], __float_as_uint(mv[1]));
+                    atomicMax(&smax.v[2], __float_as_uint(mv[2]));
                 }
             }
         }
-        if (v == ACC(0)) continue;
+        __syncthreads();
+        // exclusive scan of cnt over NC <= 512 cells: two cells per thread
+        {
+            const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
+            const int a0 = c0 < NC ? cnt[c0] : 0, a1 = c1 < NC ? cnt[c1] : 0;
+            int total;
+            const int ex = block_excl_scan_fast(a0 + a1, scan_sh, total);
+            if (c0 < NC) cst[c0] = ex;
+            if (c1 < NC) cst[c1] = ex + a0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < DEP_PER; ++k) {
+            if (lc[k] >= 0) {
+                const int j = cst[lc[k]] + rk[k];
+                skey[j] = lc[k];
+                sp[j] = qd[k];
+                if constexpr (KIND > 0) sv[j] = qv[k];
+            }
+        }
+    }
+    __syncthreads();
+    // B. lane runs of DEP_PER consecutive sorted particles, held in registers (the node
+    // window then takes over their LDS)
+    int key[DEP_PER];
+    float4 pd[DEP_PER], pv[DEP_PER];
+    const int j0 = DEP_PER * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < DEP_PER; ++k) {
+        const int j = j0 + k;
+        key[k] = j < n ? skey[j] : -1;
+        pd[k] = j < n ? sp[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (KIND > 0) pv[k] = j < n ? sv[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const DepMax m = smax;
+    __syncthreads();
+    for (int i = threadIdx.x; i < C * NW3; i += DEP_WG) nodes[i] = 0ull;
+    int ckey = -1;  // the run's last cell
+#pragma unroll
+    for (int k = 0; k < DEP_PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
+    const int next = __shfl_down(ckey, 1);
+    const int prev = __shfl_up(ckey, 1);
+    const bool any_shared = __ballot(lane > 0 && ckey >= 0 && prev == ckey) != 0ull;
+    const bool carry_end = ckey >= 0 && (lane == 63 || next != ckey);
+    __syncthreads();
+    double sc[C];
+#define LF2MI_DEP_COMP(K, I)                                                                               \
+    sc[I] = FX ? 1.0 : window_scale<K, I>(n, m);                                                           \
+    deposit_component<K, I, ACC, BC>(A, B, NWE, key, pd, pv, ckey, any_shared, carry_end, nodes, sc[I]);
+    if constexpr (KIND == 0) {
+        LF2MI_DEP_COMP(0, 0)
+    } else if constexpr (KIND == 1) {
+        LF2MI_DEP_COMP(1, 0)
+        LF2MI_DEP_COMP(1, 1)
+        LF2MI_DEP_COMP(1, 2)
+    } else {
+        LF2MI_DEP_COMP(2, 0)
+        LF2MI_DEP_COMP(2, 1)
+        LF2MI_DEP_COMP(2, 2)
+        LF2MI_DEP_COMP(2, 3)
+        LF2MI_DEP_COMP(2, 4)
+        LF2MI_DEP_COMP(2, 5)
+    }
+#undef LF2MI_DEP_COMP
+    __syncthreads();
+    // C. the node window into the field, component-fastest (consecutive addresses of the
+    // interleaved field), zeros skipped
+    const int nn = NWE * NWE * NWE;
+    for (int i = threadIdx.x; i < C * nn; i += DEP_WG) {
+        const int mnode = i / C, c = i - mnode * C;
+        const unsigned long long w = nodes[c * NW3 + mnode];
+        if (w == 0ull) continue;
+        const int ax = mnode % NWE + LO, ay = (mnode / NWE) % NWE + LO, az = mnode / (NWE * NWE) + LO;
         int gx = r.ox + ax, gy = r.oy + ay, gz = r.oz + az;
         if (A.tgt == LF2MI_DEP_PERIODIC) {
             gx = wrapi(gx, N);
             gy = wrapi(gy, N);
             gz = wrapi(gz, N);
         }
-        AccOps<ACC>::flush(A, site_index(N, A.h, gx, gy, gz) * KindC<KIND>::C + COMP, v);
-    }
-    __syncthreads();
-}
-
-// grid: one workgroup per item for KIND 0, two per item (halves of ITEM_P/2) otherwise
-template <int KIND, typename ACC>
-__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
-    constexpr int SUBP = KindC<KIND>::SUBP;
-    constexpr int PER = SUBP / DEP_WG;
-    constexpr int SPLIT = ITEM_P / SUBP;
-    constexpr int NCMAX = MAXB * MAXB * MAXB;
-    // scalar: the per-cell slots reuse the sorted particles' LDS
-    constexpr bool ALIAS = KIND == 0;
-    constexpr size_t SP_B = sizeof(float4) * SUBP;
-    constexpr size_t SV_B = KIND > 0 ? sizeof(float4) * SUBP : 0;
-    constexpr size_t CUBE_B = sizeof(ACC) * NCMAX * KindC<KIND>::NSMAX;
-    constexpr size_t POOL_B = ALIAS ? (SP_B > CUBE_B ? SP_B : CUBE_B) : SP_B + SV_B + CUBE_B;
-    __shared__ int cnt[NCMAX];
-    __shared__ int cst[NCMAX + 1];
-    __shared__ int scan_sh[DEP_WG];
-    __shared__ __align__(16) unsigned char pool[POOL_B];
-    float4 *sp = reinterpret_cast<float4 *>(pool);
-    float4 *sv = reinterpret_cast<float4 *>(pool + SP_B);
-    ACC *cube = reinterpret_cast<ACC *>(ALIAS ? pool : pool + SP_B + SV_B);
-    ItemRange r;
-    if (!item_range(blockIdx.x / SPLIT, A.start, A.istart, A.item_bucket, A.g, r)) return;
-    const int p0 = r.p0 + (int)(blockIdx.x % SPLIT) * SUBP;
-    const int p1 = min(p0 + SUBP, r.p1);
-    if (p0 >= p1) return;
-    const int B = A.g.B, N = A.g.N;
-    const int NC = B * B * B;
-    for (int i = threadIdx.x; i < NC; i += DEP_WG) cnt[i] = 0;
-    __syncthreads();
-    // A. load and rank by cell
-    int lc[PER], rk[PER];
-    float4 pd[PER], pv[PER];
+        const int64_t idx = site_index(N, A.h, gx, gy, gz) * C + c;
+        if constexpr (FX) {
+            AccOps<ACC>::flush(A, idx, w);
+        } else {
+            double inv = 1.0;
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int p = p0 + threadIdx.x + k * DEP_WG;
-        lc[k] = -1;
-        if (p < p1) {
-            const int cell = A.s.cell[p];
-            int lx, ly, lz;
-            decode_cell(cell, A.g, lx, ly, lz);
-            lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
-            pd[k] = make_float4(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p],
-                                A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv);
-            if constexpr (KIND > 0) pv[k] = make_float4(A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p], 0.0f);
-            rk[k] = atomicAdd(&cnt[lc[k]], 1);
+            for (int k = 0; k < C; ++k)
+                if (k == c) inv = 1.0 / sc[k];
+            atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
         }
-    }
-    __syncthreads();
-    // exclusive scan of cnt over NC <= 512 cells: two cells per thread
-    {
-        const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
-        const int a0 = c0 < NC ? cnt[c0] : 0, a1 = c1 < NC ? cnt[c1] : 0;
-        int total;
-        const int ex = block_excl_scan_fast(a0 + a1, scan_sh, total);
-        if (c0 < NC) cst[c0] = ex;
-        if (c1 < NC) cst[c1] = ex + a0;
-        if (threadIdx.x == 0) cst[NC] = total;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        if (lc[k] >= 0) {
-            const int j = cst[lc[k]] + rk[k];
-            sp[j] = pd[k];
-            if constexpr (KIND > 0) sv[j] = pv[k];
-        }
-    }
-    __syncthreads();
-    if constexpr (KIND == 0) {
-        deposit_component<0, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-    } else if constexpr (KIND == 1) {
-        deposit_component<1, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<1, 1, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<1, 2, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-    } else {
-        deposit_component<2, 0, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 1, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 2, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 3, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 4, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
-        deposit_component<2, 5, ACC, ALIAS>(A, r, NC, cst, sp, sv, cube);
     }
 }
 
@@ -913,6 +1057,7 @@ __device__ __forceinline__ void gkd_hist_flush(const GkdArgs &A, int b, const in
     atomicAdd(&A.next_counts[bx + NB * (by + NB * bz)], h27[t]);
 }
 
+template <int BC>
 __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     constexpr int PW = MAXB + 3;  // phi window [o-1, o+B+1]
     constexpr int FW = MAXB + 1;  // force nodes [o, o+B]
@@ -922,7 +1067,7 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     __shared__ int h27[27];
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
-    const int N = A.g.N, B = A.g.B;
+    const int N = A.g.N, B = BC ? BC : A.g.B;
     if (threadIdx.x == 0) red = 0u;
     if (threadIdx.x < 27) h27[threadIdx.x] = 0;
     __syncthreads();
@@ -1755,7 +1900,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         A.fxscale = 4294967296.0 / mref;
         {
             LF2MI_PROF("deposit_f32_fixed64", st);
-            deposit_f32_kernel<0, unsigned long long><<<g, DEP_WG, 0, st>>>(A);
+            if (p->g.B == 8) deposit_f32_kernel<0, unsigned long long, 8><<<g, DEP_WG, 0, st>>>(A);
+            else deposit_f32_kernel<0, unsigned long long, 0><<<g, DEP_WG, 0, st>>>(A);
             LF2MI_LAUNCHED();
         }
         {
@@ -1767,9 +1913,17 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         return LF2MI_OK;
     }
     LF2MI_PROF(kind == 0 ? "deposit_f32_scalar" : (kind == 1 ? "deposit_f32_vector" : "deposit_f32_vecvec"), st);
-    if (kind == 0) deposit_f32_kernel<0, float><<<g, DEP_WG, 0, st>>>(A);
-    else if (kind == 1) deposit_f32_kernel<1, float><<<2 * g, DEP_WG, 0, st>>>(A);
-    else deposit_f32_kernel<2, float><<<2 * g, DEP_WG, 0, st>>>(A);
+    const bool b8 = p->g.B == 8;
+    if (kind == 0) {
+        if (b8) deposit_f32_kernel<0, float, 8><<<g, DEP_WG, 0, st>>>(A);
+        else deposit_f32_kernel<0, float, 0><<<g, DEP_WG, 0, st>>>(A);
+    } else if (kind == 1) {
+        if (b8) deposit_f32_kernel<1, float, 8><<<g, DEP_WG, 0, st>>>(A);
+        else deposit_f32_kernel<1, float, 0><<<g, DEP_WG, 0, st>>>(A);
+    } else {
+        if (b8) deposit_f32_kernel<2, float, 8><<<g, DEP_WG, 0, st>>>(A);
+        else deposit_f32_kernel<2, float, 0><<<g, DEP_WG, 0, st>>>(A);
+    }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
@@ -1823,7 +1977,8 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
     if (vmax2) LF2MI_HIP(hipMemsetAsync(p->vmax_f32, 0, sizeof(unsigned int), st));
     {
     LF2MI_PROF("gkd_f32", st);
-    gkd_f32_kernel<<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
+    if (p->g.B == 8) gkd_f32_kernel<8><<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
+    else gkd_f32_kernel<0><<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
     LF2MI_LAUNCHED();
     }
     if (int rc = sort_into_other(p, p->cur, p->np, st)) return rc;
