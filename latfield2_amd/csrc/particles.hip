@@ -939,6 +939,7 @@ struct GkdArgs {
     double dt_drift;
     int *next_counts;       // histogram of the new buckets
     unsigned int *vmax2;    // bits of max |v|^2 (non-negative float)
+    int chunked;            // bricks below GKD_DIRECT_MAX are left to gkd_chunk_kernel
 };
 
 // kick with the gathered acceleration, drift in FP64 cell units, re-anchor and store:
@@ -979,13 +980,12 @@ __device__ __forceinline__ int kick_drift_store(const GkdArgs &A, int p, int cx,
     return newcell;
 }
 
-// CIC gather of F = -grad phi straight from phi (32 values around the cell)
-__device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, int cz, const float u[3], float a[3]) {
-    const int N = A.g.N, h = A.h;
-    const int xs[4] = {wrapi(cx - 1, N), cx, wrapi(cx + 1, N), wrapi(cx + 2, N)};
-    const int ys[4] = {wrapi(cy - 1, N), cy, wrapi(cy + 1, N), wrapi(cy + 2, N)};
-    const int zs[4] = {wrapi(cz - 1, N), cz, wrapi(cz + 1, N), wrapi(cz + 2, N)};
-    auto P = [&](int x, int y, int z) { return A.phi[site_index(N, h, x, y, z)]; };
+// CIC interpolation of F = -grad phi (centred differences, N/2 per unit of phi) at a
+// particle with offsets u in the cell whose stencil coordinates per axis are
+// xs = {x-1, x, x+1, x+2} (likewise ys, zs): the 32 values around the cell from P(x, y, z)
+template <typename PHI>
+__device__ __forceinline__ void cic_force(const PHI &P, const int xs[4], const int ys[4], const int zs[4], float hn,
+                                          const float u[3], float a[3]) {
     float core[2][2][2], xw[2][2][2], yw[2][2][2], zw[2][2][2];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
@@ -998,7 +998,6 @@ __device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, 
                 yw[k][j][i] = P(xs[1 + i], ys[j ? 3 : 0], zs[1 + k]);  // j selects y-1 / y+2
                 zw[k][j][i] = P(xs[1 + i], ys[1 + j], zs[k ? 3 : 0]);  // k selects z-1 / z+2
             }
-    const float hn = 0.5f * (float)N;
     a[0] = a[1] = a[2] = 0.0f;
 #pragma unroll
     for (int k = 0; k < 2; ++k)
@@ -1017,9 +1016,20 @@ __device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, 
             }
 }
 
-// items with fewer particles gather phi straight from HBM (through L1) instead of staging
-// the brick's window and force table in LDS, whose fixed cost only pays off for fuller
-// bricks (measured at 512^3 on clustered particles: 192 beats 64 and 512)
+// the stencil straight from phi in HBM (periodic wrap)
+__device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, int cz, const float u[3], float a[3]) {
+    const int N = A.g.N, h = A.h;
+    const int xs[4] = {wrapi(cx - 1, N), cx, wrapi(cx + 1, N), wrapi(cx + 2, N)};
+    const int ys[4] = {wrapi(cy - 1, N), cy, wrapi(cy + 1, N), wrapi(cy + 2, N)};
+    const int zs[4] = {wrapi(cz - 1, N), cz, wrapi(cz + 1, N), wrapi(cz + 2, N)};
+    const float *phi = A.phi;
+    cic_force([=](int x, int y, int z) { return phi[site_index(N, h, x, y, z)]; }, xs, ys, zs, 0.5f * (float)N, u, a);
+}
+
+// bricks with fewer particles go to the chunks of gkd_chunk_kernel (whose window serves
+// several bricks); items below it that belong to fuller bricks (a brick's last item)
+// gather phi straight from HBM instead of staging a window and force table for a few
+// particles
 static constexpr int GKD_DIRECT_MAX = 192;
 
 // histogram of the new buckets of one item's particles (bucket b): particles that stay in
@@ -1067,6 +1077,7 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     __shared__ int h27[27];
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
+    if (A.chunked && A.start[r.b + 1] - A.start[r.b] < GKD_DIRECT_MAX) return;
     const int N = A.g.N, B = BC ? BC : A.g.B;
     if (threadIdx.x == 0) red = 0u;
     if (threadIdx.x < 27) h27[threadIdx.x] = 0;
@@ -1154,6 +1165,155 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     __syncthreads();
     gkd_hist_flush(A, r.b, h27);
     if (A.vmax2 && threadIdx.x == 0) atomicMax(A.vmax2, red);
+}
+
+// ---- sparse bricks: chunks along x
+// Runs of consecutive non-empty bricks of one brick row (same y, z) whose counts are below
+// GKD_DIRECT_MAX form chunks of at most GKD_CHUNK_B bricks and ITEM_P particles (their
+// particles are contiguous in the bucket order).  A chunk stages one phi window of
+// (8 nbx + 3) x 11 x 11 nodes in LDS, so the phi lines its particles need are fetched once
+// per chunk rather than once per particle stencil (the L1 gathers of the per-particle
+// path are bound by the number of lines they touch).
+static constexpr int GKD_CHUNK_B = 8;
+static constexpr int GKD_STAGE = 8;
+
+// walk one brick row, calling emit(p0, p1, bx0, nbx) per chunk
+template <typename F>
+__device__ __forceinline__ void gkd_walk_row(const int *__restrict__ start, int64_t b_row, int NB, F emit) {
+    int bx0 = -1, p0 = 0, p1 = 0;
+    for (int bx = 0; bx < NB; ++bx) {
+        const int s0 = start[b_row + bx], s1 = start[b_row + bx + 1];
+        const int c = s1 - s0;
+        const bool take = c > 0 && c < GKD_DIRECT_MAX;
+        if (bx0 >= 0 && (!take || bx - bx0 >= GKD_CHUNK_B || p1 - p0 + c > ITEM_P)) {
+            emit(p0, p1, bx0, bx - bx0);
+            bx0 = -1;
+        }
+        if (!take) continue;
+        if (bx0 < 0) {
+            bx0 = bx;
+            p0 = s0;
+        }
+        p1 = s1;
+    }
+    if (bx0 >= 0) emit(p0, p1, bx0, NB - bx0);
+}
+
+// one thread per brick row: count the row's chunks, reserve room, write them
+// (int4: particle range, first bucket, number of bricks)
+__global__ void gkd_chunks_kernel(const int *__restrict__ start, Geo g, int4 *__restrict__ chunks,
+                                  int *__restrict__ nchunk) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (int64_t)g.NB * g.NB) return;
+    const int64_t b_row = row * g.NB;
+    int n = 0;
+    gkd_walk_row(start, b_row, g.NB, [&](int, int, int, int) { ++n; });
+    if (n == 0) return;
+    int k = atomicAdd(nchunk, n);
+    gkd_walk_row(start, b_row, g.NB, [&](int p0, int p1, int bx0, int nbx) {
+        chunks[k++] = make_int4(p0, p1, (int)(b_row + bx0), nbx);
+    });
+}
+
+// persistent over the chunk list; every workgroup leaves once the list is done
+template <int BC>
+__global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4 *__restrict__ chunks,
+                                                           int *__restrict__ nchunk) {
+    constexpr int WXM = GKD_CHUNK_B * MAXB + 3, WYM = MAXB + 3;
+    constexpr int NHM = (GKD_CHUNK_B + 2) * 9;
+    __shared__ float win[WXM * WYM * WYM];
+    __shared__ int hist[NHM];
+    __shared__ unsigned int red;
+    __shared__ int claim;
+    const int N = A.g.N, B = BC ? BC : A.g.B, NB = A.g.NB;
+    const int total = nchunk[0];
+    const float hn = 0.5f * (float)N;
+    float vmax = 0.0f;
+    if (threadIdx.x == 0) red = 0u;
+    // chunks are claimed one at a time from nchunk[1] (they hold 1 to GKD_CHUNK_B bricks)
+    for (;;) {
+        if (threadIdx.x == 0) claim = atomicAdd(&nchunk[1], 1);
+        __syncthreads();
+        const int ci = claim;
+        if (ci >= total) break;
+        const int4 ch = chunks[ci];
+        const int b0 = ch.z, nbx = ch.w;
+        const int bx0 = b0 % NB, by = (b0 / NB) % NB, bz = b0 / (NB * NB);
+        const int ox = bx0 * B, oy = by * B, oz = bz * B;
+        const int wx = nbx * B + 3, wy = B + 3, nh = (nbx + 2) * 9;
+        for (int i = threadIdx.x; i < nh; i += DEP_WG) hist[i] = 0;
+        // the first batch of particle data in flight while the window loads
+        int p = ch.x + threadIdx.x;
+        int cell = 0;
+        float u[3] = {0.f, 0.f, 0.f}, vin[3] = {0.f, 0.f, 0.f};
+        if (p < ch.y) {
+            cell = A.s.cell[p];
+            u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
+            vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
+        }
+        // GKD_STAGE loads in flight per lane before their LDS stores (a load-store loop
+        // would wait out one HBM round trip per element)
+        const int nw = wx * wy * wy;
+        for (int i0 = threadIdx.x; i0 < nw; i0 += GKD_STAGE * DEP_WG) {
+            float v[GKD_STAGE];
+#pragma unroll
+            for (int k = 0; k < GKD_STAGE; ++k) {
+                const int i = i0 + k * DEP_WG;
+                const int x = i % wx, y = (i / wx) % wy, z = i / (wx * wy);
+                v[k] = i < nw ? A.phi[site_index(N, A.h, wrapi(ox - 1 + x, N), wrapi(oy - 1 + y, N), wrapi(oz - 1 + z, N))]
+                              : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < GKD_STAGE; ++k)
+                if (i0 + k * DEP_WG < nw) win[i0 + k * DEP_WG] = v[k];
+        }
+        __syncthreads();
+        for (; p < ch.y; p += DEP_WG) {
+            if (p != ch.x + threadIdx.x) {
+                cell = A.s.cell[p];
+                u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
+                vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
+            }
+            int cx, cy, cz;
+            decode_cell(cell, A.g, cx, cy, cz);
+            // window coordinates: cell c sits at c - o + 1
+            const int lx = cx - ox + 1, ly = cy - oy + 1, lz = cz - oz + 1;
+            const int xs[4] = {lx - 1, lx, lx + 1, lx + 2};
+            const int ys[4] = {ly - 1, ly, ly + 1, ly + 2};
+            const int zs[4] = {lz - 1, lz, lz + 1, lz + 2};
+            float a[3];
+            cic_force([&](int x, int y, int z) { return win[x + wx * (y + wy * z)]; }, xs, ys, zs, hn, u, a);
+            const int newcell = kick_drift_store(A, p, cx, cy, cz, u, vin, a, vmax);
+            // new bucket relative to the chunk: x in [-1, nbx], y and z in [-1, 1] (periodic)
+            int nx, ny, nz;
+            decode_cell(newcell, A.g, nx, ny, nz);
+            nx /= B;
+            ny /= B;
+            nz /= B;
+            int dx = (nx - bx0 + NB) % NB, dy = (ny - by + NB) % NB, dz = (nz - bz + NB) % NB;
+            if (dx > nbx) dx -= NB;
+            if (dy > 1) dy -= NB;
+            if (dz > 1) dz -= NB;
+            if (dx >= -1 && dy >= -1 && dz >= -1 && dy <= 1 && dz <= 1)
+                atomicAdd(&hist[(dx + 1) + (nbx + 2) * ((dy + 1) + 3 * (dz + 1))], 1);
+            else
+                atomicAdd(&A.next_counts[nx + NB * (ny + NB * nz)], 1);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nh; i += DEP_WG) {
+            const int c = hist[i];
+            if (c == 0) continue;
+            const int ex = i % (nbx + 2) - 1, ey = (i / (nbx + 2)) % 3 - 1, ez = i / (3 * (nbx + 2)) - 1;
+            const int bx = (bx0 + ex + NB) % NB, bby = (by + ey + NB) % NB, bbz = (bz + ez + NB) % NB;
+            atomicAdd(&A.next_counts[bx + NB * (bby + NB * bbz)], c);
+        }
+        __syncthreads();
+    }
+    if (A.vmax2) {
+        atomicMax(&red, __float_as_uint(vmax));
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(A.vmax2, red);
+    }
 }
 
 // ---------------------------------------------------------------- F64 plain path
@@ -1379,6 +1539,8 @@ struct lf2mi_parts_s {
     int cur;
     lf2mi::Soa64 b64;
     int *bstart, *bcount, *istart, *bsum, *item_bucket;
+    int4 *chunks;  // gather chunks of sparse bricks (at most one per brick)
+    int *nchunk;   // [number of chunks, claim counter]
     unsigned long long *fx;
     int64_t fx_n;
     unsigned int *vmax_f32;
@@ -1522,6 +1684,8 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         A((void **)&p->bsum, 2 * 4096 * sizeof(int));
         A((void **)&p->item_bucket, (size_t)(p->g.nbk + cap / ITEM_P + 1) * sizeof(int));
         A((void **)&p->vmax_f32, sizeof(unsigned int));
+        A((void **)&p->chunks, (size_t)p->g.nbk * sizeof(int4));
+        A((void **)&p->nchunk, 2 * sizeof(int));
     } else {
         Soa64 &s = p->b64;
         A((void **)&s.cell, cap * sizeof(int32_t));
@@ -1974,12 +2138,29 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
     A.dt_drift = dt_drift;
     A.next_counts = p->bcount;
     A.vmax2 = vmax2 ? p->vmax_f32 : nullptr;
+    A.chunked = phi != nullptr;
     if (vmax2) LF2MI_HIP(hipMemsetAsync(p->vmax_f32, 0, sizeof(unsigned int), st));
+    const bool b8 = p->g.B == 8;
+    if (A.chunked) {
+        if (int rc = fill_int(p->nchunk, 2, 0, st)) return rc;
+        LF2MI_PROF("gkd_chunk_list", st);
+        const int64_t rows = (int64_t)p->g.NB * p->g.NB;
+        gkd_chunks_kernel<<<grid1(rows, 64), 64, 0, st>>>(p->bstart, p->g, p->chunks, p->nchunk);
+        LF2MI_LAUNCHED();
+    }
     {
     LF2MI_PROF("gkd_f32", st);
-    if (p->g.B == 8) gkd_f32_kernel<8><<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
+    if (b8) gkd_f32_kernel<8><<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
     else gkd_f32_kernel<0><<<(unsigned)item_grid(p), DEP_WG, 0, st>>>(A);
     LF2MI_LAUNCHED();
+    }
+    if (A.chunked) {
+        // persistent: a few workgroups per CU walk the chunk list
+        LF2MI_PROF("gkd_f32_chunks", st);
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(p->g.nbk, 2048));
+        if (b8) gkd_chunk_kernel<8><<<grid, DEP_WG, 0, st>>>(A, p->chunks, p->nchunk);
+        else gkd_chunk_kernel<0><<<grid, DEP_WG, 0, st>>>(A, p->chunks, p->nchunk);
+        LF2MI_LAUNCHED();
     }
     if (int rc = sort_into_other(p, p->cur, p->np, st)) return rc;
     if (vmax2) {
