@@ -33,14 +33,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(name: str, N: int, Np: int, esz: int) -> float | None:
-    """compulsory HBM bytes of one launch of `name` (reads + writes of its data)"""
+GKD_DIRECT_MAX = 192  # bricks below this count go to the gather's chunk kernel
+BRICK = 8
+
+
+def algorithmic_bytes(name: str, N: int, Np: int, esz: int, np_sparse: int = 0) -> float | None:
+    """compulsory HBM bytes of one launch of `name` (reads + writes of its data); the
+    gather's item kernel takes the particles of bricks with at least GKD_DIRECT_MAX of
+    them and reads phi once, its chunk kernel the other np_sparse particles"""
     M = (N // 2 + 1) * N * N
     c = 2 * esz  # complex element
     table = {
         "deposit_f32_scalar": 16 * Np + 4 * N ** 3,
         "deposit_f64_scalar": 24 * Np + 8 * N ** 3,
-        "gkd_f32": 4 * N ** 3 + 56 * Np,
+        "gkd_f32": 4 * N ** 3 + 56 * (Np - np_sparse),
+        "gkd_f32_chunks": 56 * np_sparse,
         "gkd_f64": 8 * N ** 3 + 96 * Np,
         "fft_x_r2c": esz * N ** 3 + c * M,
         "fft_x_c2r": c * M + esz * N ** 3,
@@ -60,6 +67,13 @@ def make_workload(lf, N, Np, prec, seed=3):
     rho, phi = lf.Field(lat, 1, prec), lf.Field(lat, 1, prec)
     plan = lf.PlanFFT(rho, None)
     return parts, rho, phi, plan
+
+
+def sparse_particles(torch, parts, N: int) -> int:
+    """particles in bricks of fewer than GKD_DIRECT_MAX (the gather's chunk kernel)"""
+    nb = N // BRICK
+    c = parts.cell_counts().reshape(nb, BRICK, nb, BRICK, nb, BRICK).to(torch.int64).sum(dim=(1, 3, 5))
+    return int(c[(c > 0) & (c < GKD_DIRECT_MAX)].sum().item())
 
 
 def timed_steps(torch, lf, parts, plan, rho, phi, steps, warmup, dt):
@@ -146,6 +160,7 @@ def main():
     _lib.profile_enable(False)
     prof = _lib.profile_read()
     log(f"FP32 step {ms:.3f} ms (wall {wall_ms:.3f} ms)")
+    np_sparse = sparse_particles(torch, parts, N)
     # per-kernel launch averages, from the timed steps only (warmup launches included in
     # the registry are removed by scaling with the launch counts of the timed region)
     kernels = {}
@@ -154,17 +169,22 @@ def main():
         kernels[name] = {"avg_ms": tot / cnt, "launches_per_step": per_step,
                          "ms_per_step": tot / (args.steps + args.warmup)}
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    ab = algorithmic_bytes(dom, N, Np, 4)
+    ab = algorithmic_bytes(dom, N, Np, 4, np_sparse)
     avg = kernels[dom]["avg_ms"]
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(ab / (avg * 1e-3) / 1e9, 1) if ab else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ab / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ab else None,
             "traffic": None, "avg_launch_ms": round(avg, 4), "algorithmic_bytes_per_launch": ab}
     for k, v in kernels.items():
-        b = algorithmic_bytes(k, N, Np, 4)
+        b = algorithmic_bytes(k, N, Np, 4, np_sparse)
         v["GBps"] = round(b / (v["avg_ms"] * 1e-3) / 1e9, 1) if b else None
         v["avg_ms"] = round(v["avg_ms"], 4)
         v["ms_per_step"] = round(v["ms_per_step"], 4)
+    # the gather + kick + drift stage as a whole (chunk list, item kernel, chunk kernel)
+    gk = [k for k in ("gkd_chunk_list", "gkd_f32", "gkd_f32_chunks") if k in kernels]
+    gms = sum(kernels[k]["ms_per_step"] for k in gk)
+    gather_stage = {"kernels": gk, "ms_per_step": round(gms, 4), "sparse_particles": np_sparse,
+                    "GBps": round((4 * N ** 3 + 56 * Np) / (gms * 1e-3) / 1e9, 1) if gms else None}
     # the clustered deposit timed both ways on the same particles: FP32 float atomics and
     # int64 fixed point (2^-32 units, run-to-run reproducible, converted to FP32 after)
     dep = {}
@@ -219,7 +239,7 @@ def main():
         cpu = cpu_baseline(lf, torch, threads)
         log(f"CPU baseline {cpu['value']} ms (scaled)")
 
-    step_bytes = sum((algorithmic_bytes(k, N, Np, 4) or 0) * v["launches_per_step"]
+    step_bytes = sum((algorithmic_bytes(k, N, Np, 4, np_sparse) or 0) * v["launches_per_step"]
                      for k, v in kernels.items() if k != "rebucket_scatter" and k != "zero")
     out = {
         "metric": METRIC,
@@ -242,6 +262,7 @@ def main():
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "step_frac_hbm_peak": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernels,
+        "gather_stage": gather_stage,
         "clustered_deposit": dep,
         **extra,
     }
