@@ -56,6 +56,11 @@ def _check_state(parts, N, tag):
     return d
 
 
+def _brick_edge(N):
+    """the container's brick edge: the largest divisor of N up to 8"""
+    return next(b for b in range(8, 0, -1) if N % b == 0)
+
+
 def _bricks(cell, N, B=8):
     x, y, z = cell % N, (cell // N) % N, cell // (N * N)
     return x // B + (N // B) * (y // B + (N // B) * (z // B))
@@ -70,9 +75,12 @@ def _oracle_kick(d, phi, N, dt):
     return v
 
 
-def test_steps_with_movers_match_oracle():
+@pytest.mark.parametrize("N,n", [(32, 40000),   # ~600 per brick: item path
+                                 (32, 6000),    # ~90 per brick: chunks of sparse bricks
+                                 (20, 6000),    # brick edge 5, 4^3 bricks
+                                 (12, 1000)])   # brick edge 6, 2^3 bricks (periodic images coincide)
+def test_steps_with_movers_match_oracle(N, n):
     """many brick crossings per step; every step checked on the positions it started from"""
-    N, n = 32, 40000
     announce(f"layout movers N={N} n={n}")
     rng = np.random.default_rng(5)
     pos = rng.random((n, 3))
@@ -96,7 +104,8 @@ def test_steps_with_movers_match_oracle():
         assert np.array_equal(d1["cell"], orc.bin_keys(p, N)), step
         diff = np.abs(d1["pos"] - p)
         assert np.max(np.minimum(diff, 1.0 - diff)) <= 2.0 ** -24 / N
-        moved += int(np.count_nonzero(_bricks(d1["cell"], N) != _bricks(d0["cell"], N)))
+        B = _brick_edge(N)
+        moved += int(np.count_nonzero(_bricks(d1["cell"], N, B) != _bricks(d0["cell"], N, B)))
     print(f"brick crossings over 12 steps: {moved}")
     assert moved > n  # most particles crossed a brick face at least once
     _check_state(parts, N, "final")

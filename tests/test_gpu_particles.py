@@ -91,7 +91,11 @@ def _oracle_deposits(pos, vel, mass, N, h, fold):
 @pytest.mark.parametrize("N,h,n,clustered,masses", [(16, 1, 5000, False, False), (32, 2, 40000, True, True),
                                                      (24, 1, 3000, True, False),
                                                      (64, 1, 6000, False, False),     # ~12 per brick: direct path
-                                                     (16, 1, 120000, True, True)])    # ~15k per brick: many items
+                                                     (16, 1, 120000, True, True),     # ~15k per brick: many items
+                                                     # brick edges other than 8 (N % 8 != 0): B = 5, 6, 7
+                                                     (20, 1, 8000, True, True),
+                                                     (12, 2, 20000, True, False),
+                                                     (14, 1, 500, False, False)])
 @pytest.mark.parametrize("mode", ["unfolded", "unfolded+comm", "periodic"])
 def test_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mode):
     announce(f"deposits {prec} N={N} h={h} n={n} clustered={clustered} masses={masses} mode={mode}")
@@ -168,9 +172,11 @@ def test_drift_cells_bit_exact_on_identical_velocities(prec):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
-@pytest.mark.parametrize("N,n", [(32, 30000), (64, 4000), (16, 100000)])
+@pytest.mark.parametrize("N,n", [(32, 30000), (64, 4000), (16, 100000),
+                                 (20, 8000), (12, 1500), (14, 20000)])   # B = 5, 6, 7
 def test_gather_kick_matches_oracle(prec, tol, N, n):
-    """window path (dense bricks, several items per brick) and direct path (sparse bricks)"""
+    """item path (dense bricks, several items per brick) and chunk path (runs of sparse
+    bricks), for brick edges 8 and 5-7"""
     announce(f"gather+kick {prec} N={N} n={n}")
     pos, vel, _ = _random_particles(n, 21 + N, clustered=True)
     vel *= 0.01
