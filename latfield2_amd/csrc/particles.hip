@@ -540,9 +540,11 @@ __device__ __forceinline__ bool item_range(int item, const int *__restrict__ sta
 // particle and component; the window spans nodes [LO, B] per axis, LO = -1 when a
 // component of the kind sits on links.
 template <int KIND> struct KindC;
-template <> struct KindC<0> { static constexpr int C = 1; static constexpr int LO = 0; };
-template <> struct KindC<1> { static constexpr int C = 3; static constexpr int LO = -1; };
-template <> struct KindC<2> { static constexpr int C = 6; static constexpr int LO = -1; };
+// CW: components the node window holds at a time (VecVec runs two passes of three, which
+// keeps its LDS at the vector kind's)
+template <> struct KindC<0> { static constexpr int C = 1, CW = 1, LO = 0; };
+template <> struct KindC<1> { static constexpr int C = 3, CW = 3, LO = -1; };
+template <> struct KindC<2> { static constexpr int C = 6, CW = 3, LO = -1; };
 
 // link-grid weights of one axis as a 3-vector over node offsets {-1, 0, +1}
 // (projections.hpp:1058-1071: the branch at the link midpoint)
@@ -683,7 +685,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
     constexpr int NS = SX * SY * SZ;
     constexpr int LO = KindC<KIND>::LO;
     constexpr int NWMAX = MAXB + 1 - LO;
-    unsigned long long *nd = nodes + COMP * NWMAX * NWMAX * NWMAX;
+    unsigned long long *nd = nodes + (COMP % KindC<KIND>::CW) * NWMAX * NWMAX * NWMAX;
     ACC acc[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
@@ -745,30 +747,34 @@ __device__ __forceinline__ double window_scale(int n, const DepMax &m) {
     return ldexp(1.0, 61 - e);
 }
 
-// grid: one workgroup per item
-template <int KIND, typename ACC, int BC>
+// grid: one workgroup per item.  LDS is what limits the workgroups per CU (the kernel
+// slows down steeply with fewer), so the sorted particles are kept as planes of floats
+// without a mass plane when the mass is global (PM false), with 16-bit cell keys, and the
+// cell counts turn into starts in place.
+template <int KIND, typename ACC, int BC, bool PM>
 __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
-    constexpr int C = KindC<KIND>::C;
+    constexpr int C = KindC<KIND>::C, CW = KindC<KIND>::CW;
     constexpr int LO = KindC<KIND>::LO;
     constexpr int NWMAX = MAXB + 1 - LO;
     constexpr int NW3 = NWMAX * NWMAX * NWMAX;
     constexpr int NCMAX = MAXB * MAXB * MAXB;
     constexpr bool FX = sizeof(ACC) == 8;  // global fixed point: terms quantised per particle
     static_assert(2 * DEP_WG >= NCMAX, "two cells per thread in the scan");
-    // the sorted particles, then (aliased) the node window
-    constexpr size_t SP_B = sizeof(float4) * ITEM_P;
-    constexpr size_t SV_B = KIND > 0 ? sizeof(float4) * ITEM_P : 0;
-    constexpr size_t SK_B = sizeof(int) * ITEM_P;
-    constexpr size_t ND_B = sizeof(unsigned long long) * C * NW3;
-    constexpr size_t POOL_B = SP_B + SV_B + SK_B > ND_B ? SP_B + SV_B + SK_B : ND_B;
-    __shared__ int cnt[NCMAX];
-    __shared__ int cst[NCMAX + 1];
-    __shared__ int scan_sh[DEP_WG];
+    static_assert(C % CW == 0, "whole window passes");
+    // the sorted particles (offset planes, mass plane if PM, velocity planes if KIND > 0,
+    // keys), then (aliased) the node window
+    constexpr size_t PL_B = sizeof(float) * ITEM_P;
+    constexpr int NPL = 3 + (PM ? 1 : 0) + (KIND > 0 ? 3 : 0);
+    constexpr size_t SK_B = sizeof(unsigned short) * ITEM_P;
+    constexpr size_t ND_B = sizeof(unsigned long long) * CW * NW3;
+    constexpr size_t POOL_B = NPL * PL_B + SK_B > ND_B ? NPL * PL_B + SK_B : ND_B;
+    __shared__ int cst[NCMAX];  // counts, then (in place) exclusive starts
+    __shared__ int scan_sh[DEP_WG / 64];
     __shared__ DepMax smax;
     __shared__ __align__(16) unsigned char pool[POOL_B];
-    float4 *sp = reinterpret_cast<float4 *>(pool);
-    float4 *sv = reinterpret_cast<float4 *>(pool + SP_B);
-    int *skey = reinterpret_cast<int *>(pool + SP_B + SV_B);
+    float *pl = reinterpret_cast<float *>(pool);  // plane k at pl + k * ITEM_P: u.x u.y u.z [q] [v.x v.y v.z]
+    constexpr int QPL = 3, VPL = PM ? 4 : 3;
+    unsigned short *skey = reinterpret_cast<unsigned short *>(pool + NPL * PL_B);
     unsigned long long *nodes = reinterpret_cast<unsigned long long *>(pool);
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
@@ -776,7 +782,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
     const int B = BC ? BC : A.g.B, N = A.g.N;
     const int NC = B * B * B, NWE = B + 1 - LO;
     const int lane = lane_id();
-    for (int i = threadIdx.x; i < NC; i += DEP_WG) cnt[i] = 0;
+    for (int i = threadIdx.x; i < NC; i += DEP_WG) cst[i] = 0;
     if (threadIdx.x == 0) smax = DepMax{0u, {0u, 0u, 0u}};
     __syncthreads();
     // A. load and rank by cell; item maxima of |q| and |v|
@@ -795,7 +801,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
                 decode_cell(cell, A.g, lx, ly, lz);
                 lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
                 qd[k] = make_float4(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p],
-                                    A.s.mass ? A.s.mass[p] * A.mass_nv : A.mass_nv);
+                                    PM ? A.s.mass[p] * A.mass_nv : A.mass_nv);
                 mq = fmaxf(mq, fabsf(qd[k].w));
                 if constexpr (KIND > 0) {
                     qv[k] = make_float4(A.s.vel[0][p], A.s.vel[1][p], A.s.vel[2][p], 0.0f);
@@ -803,7 +809,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
                     mv[1] = fmaxf(mv[1], fabsf(qv[k].y));
                     mv[2] = fmaxf(mv[2], fabsf(qv[k].z));
                 }
-                rk[k] = atomicAdd(&cnt[lc[k]], 1);
+                rk[k] = atomicAdd(&cst[lc[k]], 1);
             }
         }
         if constexpr (!FX) {
@@ -826,10 +832,11 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
             }
         }
         __syncthreads();
-        // exclusive scan of cnt over NC <= 512 cells: two cells per thread
+        // exclusive scan of the NC <= 512 counts in place, two cells per thread (the scan's
+        // first barrier orders every read before any write)
         {
             const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
-            const int a0 = c0 < NC ? cnt[c0] : 0, a1 = c1 < NC ? cnt[c1] : 0;
+            const int a0 = c0 < NC ? cst[c0] : 0, a1 = c1 < NC ? cst[c1] : 0;
             int total;
             const int ex = block_excl_scan_fast(a0 + a1, scan_sh, total);
             if (c0 < NC) cst[c0] = ex;
@@ -840,28 +847,56 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
         for (int k = 0; k < DEP_PER; ++k) {
             if (lc[k] >= 0) {
                 const int j = cst[lc[k]] + rk[k];
-                skey[j] = lc[k];
-                sp[j] = qd[k];
-                if constexpr (KIND > 0) sv[j] = qv[k];
+                skey[j] = (unsigned short)lc[k];
+                pl[0 * ITEM_P + j] = qd[k].x;
+                pl[1 * ITEM_P + j] = qd[k].y;
+                pl[2 * ITEM_P + j] = qd[k].z;
+                if constexpr (PM) pl[QPL * ITEM_P + j] = qd[k].w;
+                if constexpr (KIND > 0) {
+                    pl[(VPL + 0) * ITEM_P + j] = qv[k].x;
+                    pl[(VPL + 1) * ITEM_P + j] = qv[k].y;
+                    pl[(VPL + 2) * ITEM_P + j] = qv[k].z;
+                }
             }
         }
     }
     __syncthreads();
     // B. lane runs of DEP_PER consecutive sorted particles, held in registers (the node
     // window then takes over their LDS)
+    // (whole 16-byte rows of each plane: the run starts at a multiple of DEP_PER = 4; values
+    // past the item's end are masked by their key)
+    static_assert(DEP_PER == 4, "runs are read as 16-byte rows");
     int key[DEP_PER];
     float4 pd[DEP_PER], pv[DEP_PER];
     const int j0 = DEP_PER * threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < DEP_PER; ++k) {
-        const int j = j0 + k;
-        key[k] = j < n ? skey[j] : -1;
-        pd[k] = j < n ? sp[j] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (KIND > 0) pv[k] = j < n ? sv[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    {
+        const float4 ux = *reinterpret_cast<const float4 *>(pl + 0 * ITEM_P + j0);
+        const float4 uy = *reinterpret_cast<const float4 *>(pl + 1 * ITEM_P + j0);
+        const float4 uz = *reinterpret_cast<const float4 *>(pl + 2 * ITEM_P + j0);
+        float4 uq = make_float4(A.mass_nv, A.mass_nv, A.mass_nv, A.mass_nv);
+        if constexpr (PM) uq = *reinterpret_cast<const float4 *>(pl + QPL * ITEM_P + j0);
+        const ushort4 kk = *reinterpret_cast<const ushort4 *>(skey + j0);
+        pd[0] = make_float4(ux.x, uy.x, uz.x, uq.x);
+        pd[1] = make_float4(ux.y, uy.y, uz.y, uq.y);
+        pd[2] = make_float4(ux.z, uy.z, uz.z, uq.z);
+        pd[3] = make_float4(ux.w, uy.w, uz.w, uq.w);
+        key[0] = j0 + 0 < n ? (int)kk.x : -1;
+        key[1] = j0 + 1 < n ? (int)kk.y : -1;
+        key[2] = j0 + 2 < n ? (int)kk.z : -1;
+        key[3] = j0 + 3 < n ? (int)kk.w : -1;
+        if constexpr (KIND > 0) {
+            const float4 vx = *reinterpret_cast<const float4 *>(pl + (VPL + 0) * ITEM_P + j0);
+            const float4 vy = *reinterpret_cast<const float4 *>(pl + (VPL + 1) * ITEM_P + j0);
+            const float4 vz = *reinterpret_cast<const float4 *>(pl + (VPL + 2) * ITEM_P + j0);
+            pv[0] = make_float4(vx.x, vy.x, vz.x, 0.f);
+            pv[1] = make_float4(vx.y, vy.y, vz.y, 0.f);
+            pv[2] = make_float4(vx.z, vy.z, vz.z, 0.f);
+            pv[3] = make_float4(vx.w, vy.w, vz.w, 0.f);
+        }
     }
     const DepMax m = smax;
     __syncthreads();
-    for (int i = threadIdx.x; i < C * NW3; i += DEP_WG) nodes[i] = 0ull;
+    for (int i = threadIdx.x; i < CW * NW3; i += DEP_WG) nodes[i] = 0ull;
     int ckey = -1;  // the run's last cell
 #pragma unroll
     for (int k = 0; k < DEP_PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
@@ -876,45 +911,52 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
     deposit_component<K, I, ACC, BC>(A, B, NWE, key, pd, pv, ckey, any_shared, carry_end, nodes, sc[I]);
     if constexpr (KIND == 0) {
         LF2MI_DEP_COMP(0, 0)
-    } else if constexpr (KIND == 1) {
-        LF2MI_DEP_COMP(1, 0)
-        LF2MI_DEP_COMP(1, 1)
-        LF2MI_DEP_COMP(1, 2)
     } else {
-        LF2MI_DEP_COMP(2, 0)
-        LF2MI_DEP_COMP(2, 1)
-        LF2MI_DEP_COMP(2, 2)
+        LF2MI_DEP_COMP(KIND, 0)
+        LF2MI_DEP_COMP(KIND, 1)
+        LF2MI_DEP_COMP(KIND, 2)
+    }
+    __syncthreads();
+    // C. the node window into the field, component-fastest (consecutive addresses of the
+    // interleaved field), zeros skipped; components c0 .. c0 + CW - 1
+    const int nn = NWE * NWE * NWE;
+    auto flush = [&](int c0) {
+        for (int i = threadIdx.x; i < CW * nn; i += DEP_WG) {
+            const int mnode = i / CW, cw = i - mnode * CW, c = c0 + cw;
+            const unsigned long long w = nodes[cw * NW3 + mnode];
+            if (w == 0ull) continue;
+            const int ax = mnode % NWE + LO, ay = (mnode / NWE) % NWE + LO, az = mnode / (NWE * NWE) + LO;
+            int gx = r.ox + ax, gy = r.oy + ay, gz = r.oz + az;
+            if (A.tgt == LF2MI_DEP_PERIODIC) {
+                gx = wrapi(gx, N);
+                gy = wrapi(gy, N);
+                gz = wrapi(gz, N);
+            }
+            const int64_t idx = site_index(N, A.h, gx, gy, gz) * C + c;
+            if constexpr (FX) {
+                AccOps<ACC>::flush(A, idx, w);
+            } else {
+                double inv = 1.0;
+#pragma unroll
+                for (int k = 0; k < C; ++k)
+                    if (k == c) inv = 1.0 / sc[k];
+                atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
+            }
+        }
+    };
+    flush(0);
+    if constexpr (KIND == 2) {
+        // the window's second pass: T_11, T_12, T_22
+        __syncthreads();
+        for (int i = threadIdx.x; i < CW * NW3; i += DEP_WG) nodes[i] = 0ull;
+        __syncthreads();
         LF2MI_DEP_COMP(2, 3)
         LF2MI_DEP_COMP(2, 4)
         LF2MI_DEP_COMP(2, 5)
+        __syncthreads();
+        flush(3);
     }
 #undef LF2MI_DEP_COMP
-    __syncthreads();
-    // C. the node window into the field, component-fastest (consecutive addresses of the
-    // interleaved field), zeros skipped
-    const int nn = NWE * NWE * NWE;
-    for (int i = threadIdx.x; i < C * nn; i += DEP_WG) {
-        const int mnode = i / C, c = i - mnode * C;
-        const unsigned long long w = nodes[c * NW3 + mnode];
-        if (w == 0ull) continue;
-        const int ax = mnode % NWE + LO, ay = (mnode / NWE) % NWE + LO, az = mnode / (NWE * NWE) + LO;
-        int gx = r.ox + ax, gy = r.oy + ay, gz = r.oz + az;
-        if (A.tgt == LF2MI_DEP_PERIODIC) {
-            gx = wrapi(gx, N);
-            gy = wrapi(gy, N);
-            gz = wrapi(gz, N);
-        }
-        const int64_t idx = site_index(N, A.h, gx, gy, gz) * C + c;
-        if constexpr (FX) {
-            AccOps<ACC>::flush(A, idx, w);
-        } else {
-            double inv = 1.0;
-#pragma unroll
-            for (int k = 0; k < C; ++k)
-                if (k == c) inv = 1.0 / sc[k];
-            atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
-        }
-    }
 }
 
 __global__ void fixed_to_field_kernel(const unsigned long long *__restrict__ fx, float *__restrict__ f, int64_t n,
@@ -1609,6 +1651,19 @@ static int read_total(lf2mi_parts_s *p, hipStream_t st, int64_t *out) {
 
 static int64_t item_grid(const lf2mi_parts_s *p) { return p->g.nbk + p->np / ITEM_P + 1; }
 
+// deposit kernel for brick edge 8 or any, with or without per-particle masses
+template <int KIND, typename ACC>
+static void launch_deposit(const DepArgs &A, bool b8, unsigned grid, hipStream_t st) {
+    const bool pm = A.s.mass != nullptr;
+    if (b8) {
+        if (pm) deposit_f32_kernel<KIND, ACC, 8, true><<<grid, DEP_WG, 0, st>>>(A);
+        else deposit_f32_kernel<KIND, ACC, 8, false><<<grid, DEP_WG, 0, st>>>(A);
+    } else {
+        if (pm) deposit_f32_kernel<KIND, ACC, 0, true><<<grid, DEP_WG, 0, st>>>(A);
+        else deposit_f32_kernel<KIND, ACC, 0, false><<<grid, DEP_WG, 0, st>>>(A);
+    }
+}
+
 static int ic_common(lf2mi_parts_s *p, IcArgs &A, hipStream_t st) {
     if (A.np > p->cap) return fail(LF2MI_ERANGE, "%lld particles exceed the capacity %lld", (long long)A.np, (long long)p->cap);
     if (p->dtype == LF2MI_F32) {
@@ -2064,8 +2119,7 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         A.fxscale = 4294967296.0 / mref;
         {
             LF2MI_PROF("deposit_f32_fixed64", st);
-            if (p->g.B == 8) deposit_f32_kernel<0, unsigned long long, 8><<<g, DEP_WG, 0, st>>>(A);
-            else deposit_f32_kernel<0, unsigned long long, 0><<<g, DEP_WG, 0, st>>>(A);
+            launch_deposit<0, unsigned long long>(A, p->g.B == 8, g, st);
             LF2MI_LAUNCHED();
         }
         {
@@ -2079,14 +2133,11 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     LF2MI_PROF(kind == 0 ? "deposit_f32_scalar" : (kind == 1 ? "deposit_f32_vector" : "deposit_f32_vecvec"), st);
     const bool b8 = p->g.B == 8;
     if (kind == 0) {
-        if (b8) deposit_f32_kernel<0, float, 8><<<g, DEP_WG, 0, st>>>(A);
-        else deposit_f32_kernel<0, float, 0><<<g, DEP_WG, 0, st>>>(A);
+        launch_deposit<0, float>(A, b8, g, st);
     } else if (kind == 1) {
-        if (b8) deposit_f32_kernel<1, float, 8><<<g, DEP_WG, 0, st>>>(A);
-        else deposit_f32_kernel<1, float, 0><<<g, DEP_WG, 0, st>>>(A);
+        launch_deposit<1, float>(A, b8, g, st);
     } else {
-        if (b8) deposit_f32_kernel<2, float, 8><<<g, DEP_WG, 0, st>>>(A);
-        else deposit_f32_kernel<2, float, 0><<<g, DEP_WG, 0, st>>>(A);
+        launch_deposit<2, float>(A, b8, g, st);
     }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
