@@ -959,6 +959,126 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
 #undef LF2MI_DEP_COMP
 }
 
+// Scalar deposit with a global mass: a workgroup takes up to DEP_MERGE consecutive items of
+// one brick (the first of each group; the others exit) and sorts them one after the other
+// into the same node window, so a dense brick's nodes are added to the field once per group
+// rather than once per item.  With one mass the fixed-point bound is n m, known before any
+// particle is read.  Phases as in deposit_f32_kernel; the window has its own LDS here.
+static constexpr int DEP_MERGE = 8;
+
+template <typename ACC, int BC>
+__global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
+    constexpr int NWMAX = MAXB + 1;
+    constexpr int NW3 = NWMAX * NWMAX * NWMAX;
+    constexpr int NCMAX = MAXB * MAXB * MAXB;
+    constexpr bool FX = sizeof(ACC) == 8;
+    __shared__ int cst[NCMAX];
+    __shared__ int scan_sh[DEP_WG / 64];
+    __shared__ __align__(16) float pl[3][ITEM_P];
+    __shared__ __align__(16) unsigned short skey[ITEM_P];
+    __shared__ unsigned long long nodes[NW3];
+    ItemRange r;
+    if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
+    const int bstart = A.start[r.b], bend = A.start[r.b + 1];
+    if (((r.p0 - bstart) / ITEM_P) % DEP_MERGE != 0) return;  // taken by the group's first item
+    const int end = min(bend, r.p0 + DEP_MERGE * ITEM_P);
+    const int B = BC ? BC : A.g.B, N = A.g.N;
+    const int NC = B * B * B, NWE = B + 1;
+    const int lane = lane_id();
+    const DepMax m{__float_as_uint(A.mass_nv), {0u, 0u, 0u}};
+    const double sc = FX ? 1.0 : window_scale<0, 0>(end - r.p0, m);
+    for (int i = threadIdx.x; i < NW3; i += DEP_WG) nodes[i] = 0ull;
+    for (int p0 = r.p0; p0 < end; p0 += ITEM_P) {
+        const int n = min(ITEM_P, end - p0);
+        for (int i = threadIdx.x; i < NC; i += DEP_WG) cst[i] = 0;
+        __syncthreads();
+        // A. load and rank by cell
+        {
+            int lc[DEP_PER], rk[DEP_PER];
+            float3 qd[DEP_PER];
+#pragma unroll
+            for (int k = 0; k < DEP_PER; ++k) {
+                const int i = threadIdx.x + k * DEP_WG;
+                lc[k] = -1;
+                if (i < n) {
+                    const int p = p0 + i;
+                    int lx, ly, lz;
+                    decode_cell(A.s.cell[p], A.g, lx, ly, lz);
+                    lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
+                    qd[k] = make_float3(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]);
+                    rk[k] = atomicAdd(&cst[lc[k]], 1);
+                }
+            }
+            __syncthreads();
+            {
+                const int c0 = 2 * threadIdx.x, c1 = c0 + 1;
+                const int a0 = c0 < NC ? cst[c0] : 0, a1 = c1 < NC ? cst[c1] : 0;
+                int total;
+                const int ex = block_excl_scan_fast(a0 + a1, scan_sh, total);
+                if (c0 < NC) cst[c0] = ex;
+                if (c1 < NC) cst[c1] = ex + a0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < DEP_PER; ++k) {
+                if (lc[k] >= 0) {
+                    const int j = cst[lc[k]] + rk[k];
+                    skey[j] = (unsigned short)lc[k];
+                    pl[0][j] = qd[k].x;
+                    pl[1][j] = qd[k].y;
+                    pl[2][j] = qd[k].z;
+                }
+            }
+        }
+        __syncthreads();
+        // B. lane runs into the window
+        int key[DEP_PER];
+        float4 pd[DEP_PER], pv[DEP_PER];
+        const int j0 = DEP_PER * threadIdx.x;
+        {
+            const float4 ux = *reinterpret_cast<const float4 *>(&pl[0][j0]);
+            const float4 uy = *reinterpret_cast<const float4 *>(&pl[1][j0]);
+            const float4 uz = *reinterpret_cast<const float4 *>(&pl[2][j0]);
+            const ushort4 kk = *reinterpret_cast<const ushort4 *>(&skey[j0]);
+            const float q = A.mass_nv;
+            pd[0] = make_float4(ux.x, uy.x, uz.x, q);
+            pd[1] = make_float4(ux.y, uy.y, uz.y, q);
+            pd[2] = make_float4(ux.z, uy.z, uz.z, q);
+            pd[3] = make_float4(ux.w, uy.w, uz.w, q);
+            key[0] = j0 + 0 < n ? (int)kk.x : -1;
+            key[1] = j0 + 1 < n ? (int)kk.y : -1;
+            key[2] = j0 + 2 < n ? (int)kk.z : -1;
+            key[3] = j0 + 3 < n ? (int)kk.w : -1;
+        }
+        int ckey = -1;
+#pragma unroll
+        for (int k = 0; k < DEP_PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
+        const int next = __shfl_down(ckey, 1);
+        const int prev = __shfl_up(ckey, 1);
+        const bool any_shared = __ballot(lane > 0 && ckey >= 0 && prev == ckey) != 0ull;
+        const bool carry_end = ckey >= 0 && (lane == 63 || next != ckey);
+        deposit_component<0, 0, ACC, BC>(A, B, NWE, key, pd, pv, ckey, any_shared, carry_end, nodes, sc);
+        __syncthreads();  // the planes and counts are reused by the next item
+    }
+    // C. the window into the field
+    const int nn = NWE * NWE * NWE;
+    const double inv = 1.0 / sc;
+    for (int i = threadIdx.x; i < nn; i += DEP_WG) {
+        const unsigned long long w = nodes[i];
+        if (w == 0ull) continue;
+        const int ax = i % NWE, ay = (i / NWE) % NWE, az = i / (NWE * NWE);
+        int gx = r.ox + ax, gy = r.oy + ay, gz = r.oz + az;
+        if (A.tgt == LF2MI_DEP_PERIODIC) {
+            gx = wrapi(gx, N);
+            gy = wrapi(gy, N);
+            gz = wrapi(gz, N);
+        }
+        const int64_t idx = site_index(N, A.h, gx, gy, gz);
+        if constexpr (FX) AccOps<ACC>::flush(A, idx, w);
+        else atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
+    }
+}
+
 __global__ void fixed_to_field_kernel(const unsigned long long *__restrict__ fx, float *__restrict__ f, int64_t n,
                                       double scale) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1073,6 +1193,7 @@ __device__ __forceinline__ void gather_direct(const GkdArgs &A, int cx, int cy, 
 // gather phi straight from HBM instead of staging a window and force table for a few
 // particles
 static constexpr int GKD_DIRECT_MAX = 192;
+static constexpr int GKD_MERGE = 4;
 
 // histogram of the new buckets of one item's particles (bucket b): particles that stay in
 // the brick are counted with one LDS atomic per wave, moves to the 26 neighbouring bricks
@@ -1120,6 +1241,13 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
     if (A.chunked && A.start[r.b + 1] - A.start[r.b] < GKD_DIRECT_MAX) return;
+    if (A.phi) {
+        // one workgroup per GKD_MERGE consecutive items of a brick: one phi window and force
+        // table serve them all (the group's other items exit)
+        const int bs = A.start[r.b], be = A.start[r.b + 1];
+        if (((r.p0 - bs) / ITEM_P) % GKD_MERGE != 0) return;
+        r.p1 = min(be, r.p0 + GKD_MERGE * ITEM_P);
+    }
     const int N = A.g.N, B = BC ? BC : A.g.B;
     if (threadIdx.x == 0) red = 0u;
     if (threadIdx.x < 27) h27[threadIdx.x] = 0;
@@ -1655,6 +1783,11 @@ static int64_t item_grid(const lf2mi_parts_s *p) { return p->g.nbk + p->np / ITE
 template <int KIND, typename ACC>
 static void launch_deposit(const DepArgs &A, bool b8, unsigned grid, hipStream_t st) {
     const bool pm = A.s.mass != nullptr;
+    if (KIND == 0 && !pm) {
+        if (b8) deposit_merged_kernel<ACC, 8><<<grid, DEP_WG, 0, st>>>(A);
+        else deposit_merged_kernel<ACC, 0><<<grid, DEP_WG, 0, st>>>(A);
+        return;
+    }
     if (b8) {
         if (pm) deposit_f32_kernel<KIND, ACC, 8, true><<<grid, DEP_WG, 0, st>>>(A);
         else deposit_f32_kernel<KIND, ACC, 8, false><<<grid, DEP_WG, 0, st>>>(A);
