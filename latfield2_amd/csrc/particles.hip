@@ -87,68 +87,6 @@ __device__ __forceinline__ int agg_inc(int *counters, int key, bool active) {
     return slot;
 }
 
-// block-wide aggregated atomic increment (blockDim.x == BLK, every thread calls it):
-// lanes of the whole block with the same key share one global atomicAdd.  Particles are
-// processed in bucket order, so a block's keys are a handful of hot buckets; per-wave
-// aggregation alone leaves one atomic per wave on the same counter, which serialises in
-// L2 for clustered distributions.  Returns each lane's slot when want_slot.
-template <int BLK>
-struct BlockAgg {
-    int key[BLK];
-    int cnt[BLK];
-    int base[BLK];
-    int n;
-};
-
-template <int BLK>
-__device__ __forceinline__ int block_agg_inc(int *counters, int key, bool active, BlockAgg<BLK> &S) {
-    const int lane = lane_id();
-    if (threadIdx.x == 0) S.n = 0;
-    __syncthreads();
-    // per wave: groups of equal keys, each leader appends (key, count) to the block list
-    unsigned long long todo = __ballot(active);
-    int my_entry = -1, my_rank = 0;
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int lkey = __shfl(key, leader);
-        const unsigned long long grp = __ballot(active && key == lkey);
-        int e = 0;
-        if (lane == leader) {
-            e = atomicAdd(&S.n, 1);
-            S.key[e] = lkey;
-            S.cnt[e] = __popcll(grp);
-        }
-        e = __shfl(e, leader);
-        if (active && key == lkey) {
-            my_entry = e;
-            my_rank = __popcll(grp & ((1ull << lane) - 1ull));
-        }
-        todo &= ~grp;
-    }
-    __syncthreads();
-    // per entry: offset among earlier entries of the same key; the first entry of a key
-    // does the global atomic for the key's total
-    const int n = S.n;
-    int off = 0, first = -1, total = 0;
-    if ((int)threadIdx.x < n) {
-        const int k = S.key[threadIdx.x];
-        for (int e = 0; e < n; ++e) {
-            if (S.key[e] != k) continue;
-            if (first < 0) first = e;
-            if (e < (int)threadIdx.x) off += S.cnt[e];
-            total += S.cnt[e];
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < n && first == (int)threadIdx.x) S.base[threadIdx.x] = atomicAdd(&counters[S.key[threadIdx.x]], total);
-    __syncthreads();
-    if ((int)threadIdx.x < n) S.cnt[threadIdx.x] = S.base[first] + off;  // reuse cnt as the entry's base
-    __syncthreads();
-    const int slot = my_entry >= 0 ? S.cnt[my_entry] + my_rank : 0;
-    __syncthreads();
-    return slot;
-}
-
 // ---------------------------------------------------------------- geometry helpers
 struct Geo {
     int N, B, NB;
@@ -457,37 +395,97 @@ __global__ void scan_add_kernel(int *__restrict__ start, int *__restrict__ istar
 }
 
 // ---------------------------------------------------------------- scatter (counting sort)
+// Counting-sort scatter, SCAT_PER particles per thread (1024 per block): every load is in
+// flight before the slots are claimed.  Slots: per wave, lanes with equal new buckets form
+// groups (leader loop) whose leader adds the group size to the bucket's entry in an LDS hash
+// table (one CAS to insert the key); after a barrier one global atomic per distinct bucket of
+// the block reserves the range, and after a second barrier every particle stores at
+// start[b] + base + rank.  A re-bucketing block sees a few buckets (the source is bucket
+// ordered); a block of an unordered source (initial sort) at most 1024, for which the
+// 2048-slot table has room.
+static constexpr int SCAT_PER = 4;
+static constexpr int SCAT_HT = 2048;
+
 __global__ __launch_bounds__(256) void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g,
                                                       const int *__restrict__ start, int *__restrict__ cursor) {
-    __shared__ BlockAgg<256> agg;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = i < np;
-    const int cell = in ? src.cell[i] : -1;
-    const bool act = in && cell >= 0;
-    // the particle's data is in flight while the slot is claimed
-    float o[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, m = 0.f;
-    int64_t id = 0;
-    if (act) {
+    __shared__ int hkey[SCAT_HT];
+    __shared__ int hval[SCAT_HT];  // group counts, then the bucket's reserved base
+    for (int i = threadIdx.x; i < SCAT_HT; i += 256) {
+        hkey[i] = -1;
+        hval[i] = 0;
+    }
+    const int64_t i0 = (int64_t)blockIdx.x * (256 * SCAT_PER) + threadIdx.x;
+    int cell[SCAT_PER];
+    float o[SCAT_PER][3], v[SCAT_PER][3], m[SCAT_PER];
+    int64_t id[SCAT_PER];
+#pragma unroll
+    for (int k = 0; k < SCAT_PER; ++k) {
+        const int64_t i = i0 + k * 256;
+        cell[k] = i < np ? src.cell[i] : -1;
+        m[k] = 0.f;
+        id[k] = 0;
+        if (cell[k] >= 0) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                o[k][d] = src.off[d][i];
+                v[k][d] = src.vel[d][i];
+            }
+            if (src.mass) m[k] = src.mass[i];
+            if (src.id) id[k] = src.id[i];
+        }
+    }
+    __syncthreads();
+    const int lane = lane_id();
+    int bk[SCAT_PER], ent[SCAT_PER], rank[SCAT_PER];
+#pragma unroll
+    for (int k = 0; k < SCAT_PER; ++k) {
+        const bool act = cell[k] >= 0;
+        bk[k] = act ? bucket_of(cell[k], g) : 0;
+        ent[k] = 0;
+        rank[k] = 0;
+        unsigned long long todo = __ballot(act);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lkey = __shfl(bk[k], leader);
+            const unsigned long long grp = __ballot(act && bk[k] == lkey);
+            int e = 0, r0 = 0;
+            if (lane == leader) {
+                e = (int)(((unsigned)lkey * 2654435761u) >> 21) & (SCAT_HT - 1);
+                for (;;) {
+                    const int old = atomicCAS(&hkey[e], -1, lkey);
+                    if (old == -1 || old == lkey) break;
+                    e = (e + 1) & (SCAT_HT - 1);
+                }
+                r0 = atomicAdd(&hval[e], __popcll(grp));
+            }
+            e = __shfl(e, leader);
+            r0 = __shfl(r0, leader);
+            if (act && bk[k] == lkey) {
+                ent[k] = e;
+                rank[k] = r0 + __popcll(grp & ((1ull << lane) - 1ull));
+            }
+            todo &= ~grp;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < SCAT_HT; i += 256) {
+        const int key = hkey[i];
+        if (key >= 0) hval[i] = start[key] + atomicAdd(&cursor[key], hval[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAT_PER; ++k) {
+        if (cell[k] < 0) continue;
+        const int64_t j = (int64_t)hval[ent[k]] + rank[k];
+        dst.cell[j] = cell[k];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            o[d] = src.off[d][i];
-            v[d] = src.vel[d][i];
+            dst.off[d][j] = o[k][d];
+            dst.vel[d][j] = v[k][d];
         }
-        if (src.mass) m = src.mass[i];
-        if (src.id) id = src.id[i];
+        if (src.mass) dst.mass[j] = m[k];
+        if (src.id) dst.id[j] = id[k];
     }
-    const int b = act ? bucket_of(cell, g) : 0;
-    const int slot = block_agg_inc<256>(cursor, b, act, agg);
-    if (!act) return;
-    const int64_t j = (int64_t)start[b] + slot;
-    dst.cell[j] = cell;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        dst.off[d][j] = o[d];
-        dst.vel[d][j] = v[d];
-    }
-    if (src.mass) dst.mass[j] = m;
-    if (src.id) dst.id[j] = id;
 }
 
 // ---------------------------------------------------------------- items
@@ -1763,7 +1761,8 @@ static int sort_into_other(lf2mi_parts_s *p, int src, int64_t nsrc, hipStream_t 
     if (int rc = scan_buckets(p, st)) return rc;
     if (int rc = fill_int(p->bcount, p->g.nbk, 0, st)) return rc;
     LF2MI_PROF("rebucket_scatter", st);
-    scatter_kernel<<<grid1(nsrc), 256, 0, st>>>(nsrc, p->b32[src], p->b32[1 - src], p->g, p->bstart, p->bcount);
+    scatter_kernel<<<grid1(nsrc, 256 * SCAT_PER), 256, 0, st>>>(nsrc, p->b32[src], p->b32[1 - src], p->g, p->bstart,
+                                                                  p->bcount);
     LF2MI_LAUNCHED();
     p->cur = 1 - src;
     return LF2MI_OK;
