@@ -194,3 +194,40 @@ def test_fused_poisson_solve_matches_oracle_chain(prec, tol, N):
     else:
         assert np.max(np.abs(got - refi)) <= tol * np.max(np.abs(refi))
     assert np.array_equal(real_np(rho), rin)  # rho is read only
+
+
+@pytest.mark.parametrize("N", [512, 1024])
+def test_fused_poisson_solve_large_matches_unfused_chain(N):
+    """the fused solve at the benchmark sizes (1024^3 runs its workspace transposed) against
+    execute(FFT_FORWARD) -> poisson_green -> execute(FFT_BACKWARD), whose kernels are pinned
+    against the oracle at smaller N and by the round trips at 1024^3; FP32, rel-L2 <= 1e-6"""
+    free, _ = torch.cuda.mem_get_info()
+    need = 8 * (N + 2) ** 3 * 4
+    if free < need:
+        pytest.skip(f"needs {need / 2 ** 30:.0f} GiB of device memory")
+    announce(f"poisson_solve fused vs unfused f32 N={N}")
+    lat = lf.Lattice(3, N, 1)
+    rho = lf.Field(lat, 1, lf.F32)
+    g = torch.Generator(device="cuda").manual_seed(N)
+    rho.t.copy_(torch.rand(rho.t.shape, generator=g, device="cuda", dtype=torch.float32) + 2.0)
+    inner = rho.t[1:N + 1, 1:N + 1, 1:N + 1, 0]
+    mean = float(inner.double().mean().item())
+    phi = lf.Field(lat, 1, lf.F32)
+    plan = lf.PlanFFT(rho, None)
+    lf.poisson_solve(plan, rho, phi, shift=mean, scale=1.0 / N ** 3)
+    del plan
+    # the unfused chain on the mean-subtracted field (the same FP32 subtraction)
+    inner -= mean
+    k = lf.KField(lat, 0, 1, lf.F32)
+    plan2 = lf.PlanFFT(rho, k)
+    plan2.execute(lf.FFT_FORWARD)
+    lf.poisson_green(k, res=1.0 / N, scale=1.0 / N ** 3)
+    ref = lf.Field(lat, 1, lf.F32)
+    plan3 = lf.PlanFFT(ref, k)
+    plan3.execute(lf.FFT_BACKWARD)
+    torch.cuda.synchronize()
+    a = phi.t[1:N + 1, 1:N + 1, 1:N + 1, 0].double()
+    b = ref.t[1:N + 1, 1:N + 1, 1:N + 1, 0].double()
+    err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    print(f"N={N}: fused vs unfused rel-L2 {err:.2e}")
+    assert err <= 1e-6
