@@ -529,8 +529,8 @@ __device__ __forceinline__ bool item_range(int item, const int *__restrict__ sta
 //      crowded cell costs the same as a sparse one.  The window holds 64-bit fixed point
 //      (integer LDS atomics: the FP32 LDS atomic add is many times slower on gfx950): per
 //      item and component a power-of-two scale puts the bound sum_p |q_p| (from the item's
-//      maxima of |q| and |v|) just below 2^61, so the window sums are exact and the only
-//      rounding past the FP32 run sums is the quantum 2^-61 of that bound;
+//      maxima of |q| and |v|) just below 2^51, so the window sums are exact and the only
+//      rounding past the FP32 run sums is the quantum 2^-51 of that bound;
 //   C. the node window (all components, component-fastest) is added into the field with
 //      one global atomic per non-zero value (periodic wrap or the unfolded ghost layer).
 // Node offsets per axis: {0, 1} on the vertex grid and {-1, 0, +1} on the link grid
@@ -646,9 +646,12 @@ __device__ __forceinline__ void seg_scan(int key, T (&v)[NS]) {
 
 static constexpr int DEP_PER = ITEM_P / DEP_WG;  // sorted particles per lane run
 
-// window value of a run sum: FP32 sums in the item's fixed point, fixed-point terms as they are
+// window value of a run sum: FP32 sums in the item's fixed point (nearest), fixed-point terms
+// as they are.  |x scale| < 2^51, so adding 1.5 2^52 leaves the rounded integer in the low
+// mantissa bits of the sum (two's complement once the constant's bits are taken off).
 __device__ __forceinline__ unsigned long long to_window(float x, double scale) {
-    return (unsigned long long)llrint((double)x * scale);
+    const double t = fma((double)x, scale, 0x1.8p52);
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000ll);
 }
 __device__ __forceinline__ unsigned long long to_window(unsigned long long x, double) { return x; }
 
@@ -732,7 +735,7 @@ struct DepMax {
     unsigned int q, v[3];
 };
 
-// per-item fixed-point scale of one component: 2^(61 - e) with sum_p |q_p v_a v_b| < 2^e
+// per-item fixed-point scale of one component: 2^(51 - e) with sum_p |q_p v_a v_b| < 2^e
 template <int KIND, int COMP>
 __device__ __forceinline__ double window_scale(int n, const DepMax &m) {
     constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
@@ -742,7 +745,7 @@ __device__ __forceinline__ double window_scale(int n, const DepMax &m) {
     if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;  // all zero (or not finite)
     int e;
     frexp(bound, &e);
-    return ldexp(1.0, 61 - e);
+    return ldexp(1.0, 51 - e);
 }
 
 // grid: one workgroup per item.  LDS is what limits the workgroups per CU (the kernel
