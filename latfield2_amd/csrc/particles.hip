@@ -989,24 +989,37 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
     const DepMax m{__float_as_uint(A.mass_nv), {0u, 0u, 0u}};
     const double sc = FX ? 1.0 : window_scale<0, 0>(end - r.p0, m);
     for (int i = threadIdx.x; i < NW3; i += DEP_WG) nodes[i] = 0ull;
+    // the next item's particles are loaded before the current item's lane runs
+    int nc[DEP_PER];
+    float3 nq[DEP_PER];
+    auto fetch = [&](int q0) {
+#pragma unroll
+        for (int k = 0; k < DEP_PER; ++k) {
+            const int p = q0 + threadIdx.x + k * DEP_WG;
+            nc[k] = -1;
+            if (p < end) {
+                nc[k] = A.s.cell[p];
+                nq[k] = make_float3(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]);
+            }
+        }
+    };
+    fetch(r.p0);
     for (int p0 = r.p0; p0 < end; p0 += ITEM_P) {
         const int n = min(ITEM_P, end - p0);
         for (int i = threadIdx.x; i < NC; i += DEP_WG) cst[i] = 0;
         __syncthreads();
-        // A. load and rank by cell
+        // A. rank by cell
         {
             int lc[DEP_PER], rk[DEP_PER];
             float3 qd[DEP_PER];
 #pragma unroll
             for (int k = 0; k < DEP_PER; ++k) {
-                const int i = threadIdx.x + k * DEP_WG;
                 lc[k] = -1;
-                if (i < n) {
-                    const int p = p0 + i;
+                qd[k] = nq[k];
+                if (nc[k] >= 0) {
                     int lx, ly, lz;
-                    decode_cell(A.s.cell[p], A.g, lx, ly, lz);
+                    decode_cell(nc[k], A.g, lx, ly, lz);
                     lc[k] = (lx - r.ox) + B * ((ly - r.oy) + B * (lz - r.oz));
-                    qd[k] = make_float3(A.s.off[0][p], A.s.off[1][p], A.s.off[2][p]);
                     rk[k] = atomicAdd(&cst[lc[k]], 1);
                 }
             }
@@ -1051,6 +1064,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
             key[2] = j0 + 2 < n ? (int)kk.z : -1;
             key[3] = j0 + 3 < n ? (int)kk.w : -1;
         }
+        if (p0 + ITEM_P < end) fetch(p0 + ITEM_P);
         int ckey = -1;
 #pragma unroll
         for (int k = 0; k < DEP_PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
