@@ -423,58 +423,77 @@ class Particles {
         sync();
         lf2mi_or_die(lf2mi_gather_kick_drift(h_, nullptr, 1, 0.0, dtau, nullptr, nullptr), "Particles::moveParticles");
     }
-    // copy all particles back (container order)
+    // copy all particles back (container order): the whole structs as added, with pos,
+    // vel and ID brought up to date
     void getParticles(std::vector<part> &out) {
         sync();
         const long n = numParticles();
-        std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n);
-        std::vector<int64_t> id((size_t)n);
-        lf2mi_or_die(lf2mi_parts_download(h_, pos.data(), vel.data(), nullptr, id.data(), nullptr, nullptr),
-                     "Particles::getParticles");
         out.resize((size_t)n);
-        for (long i = 0; i < n; ++i) {
-            out[(size_t)i].ID = (long)id[(size_t)i];
-            for (int d = 0; d < 3; ++d) {
-                out[(size_t)i].pos[d] = (Real)pos[3 * (size_t)i + d];
-                out[(size_t)i].vel[d] = (Real)vel[3 * (size_t)i + d];
-            }
-        }
+        int64_t got = 0;
+        lf2mi_or_die(lf2mi_parts_download_aos(h_, out.data(), (int64_t)n, (int64_t)sizeof(part), lf2mi_real_code(),
+                                              (int64_t)offsetof(part, pos), (int64_t)offsetof(part, vel),
+                                              (int64_t)offsetof(part, ID), &got, nullptr),
+                     "Particles::getParticles");
+        out.resize((size_t)got);
     }
     lf2mi_parts handle() {
         sync();
         return h_;
     }
     double mass() const { return info_.mass; }
+    // the mass a projection uses (projections.hpp:110-126): without oset the one detected at
+    // initialize (a `mass` member of part, else part_info's mass, Particles.hpp:398-427);
+    // with oset the double at that byte offset of part_info (FROM_INFO) or of every
+    // particle (FROM_PART)
+    void select_mass(size_t *oset, int flag_where) {
+        sync();
+        if (oset == nullptr) {
+            if (has_mass_member<part>::value)
+                lf2mi_or_die(lf2mi_parts_mass_from(h_, LF2MI_MASS_PART, mass_offset<part>(), nullptr), "projection mass");
+            else
+                use_global_mass(info_.mass);
+        } else if (flag_where == FROM_INFO) {
+            double m;
+            std::memcpy(&m, reinterpret_cast<const char *>(&info_) + *oset, sizeof(double));
+            use_global_mass(m);
+        } else {
+            lf2mi_or_die(lf2mi_parts_mass_from(h_, LF2MI_MASS_PART, (int64_t)*oset, nullptr), "projection mass");
+        }
+    }
 
   private:
+    // the staged structs go to the device whole (a payload of sizeof(part) bytes moves with
+    // each particle), read by their members' offsets
     void sync() {
         if (!dirty_) return;
         if (h_) lf2mi_parts_destroy(h_);
         h_ = nullptr;
+        static_assert(sizeof(part) % 8 == 0, "part structs are kept whole in 8-byte words");
+        static_assert(sizeof(((part *)nullptr)->ID) == 8, "the part ID is a 64-bit long");
         const bool pm = has_mass_member<part>::value;
         lf2mi_or_die(lf2mi_parts_create(N_, (int64_t)staged_.size(), lf2mi_real_code(), LF2MI_PARTS_ID | (pm ? LF2MI_PARTS_MASS : 0), &h_),
                      "Particles::initialize");
         lf2mi_or_die(lf2mi_parts_set_mass(h_, info_.mass), "Particles::initialize");
-        const size_t n = staged_.size();
-        std::vector<double> pos(3 * n), vel(3 * n), m(pm ? n : 0);
-        std::vector<int64_t> id(n);
-        for (size_t i = 0; i < n; ++i) {
-            id[i] = staged_[i].ID;
-            for (int d = 0; d < 3; ++d) {
-                pos[3 * i + d] = staged_[i].pos[d];
-                vel[3 * i + d] = staged_[i].vel[d];
-            }
-            mass_of(staged_[i], m, i);
-        }
-        lf2mi_or_die(lf2mi_parts_upload(h_, (int64_t)n, pos.data(), vel.data(), pm ? m.data() : nullptr, id.data(),
-                                        nullptr, nullptr),
+        lf2mi_or_die(lf2mi_parts_set_payload(h_, (int)sizeof(part)), "Particles::initialize");
+        lf2mi_or_die(lf2mi_parts_upload_aos(h_, (int64_t)staged_.size(), staged_.data(), (int64_t)sizeof(part),
+                                            lf2mi_real_code(), (int64_t)offsetof(part, pos), (int64_t)offsetof(part, vel),
+                                            (int64_t)offsetof(part, ID), pm ? mass_offset<part>() : -1, nullptr, nullptr),
                      "Particles::addParticle_global");
         staged_.clear();
         dirty_ = false;
     }
     template <typename P>
-    static void mass_of(const P &p, std::vector<double> &m, size_t i) {
-        if constexpr (has_mass_member<P>::value) m[i] = (double)p.mass;
+    static int64_t mass_offset() {
+        if constexpr (has_mass_member<P>::value) {
+            static_assert(sizeof(((P *)nullptr)->mass) == sizeof(double), "a per-particle mass is a double");
+            return (int64_t)offsetof(P, mass);
+        } else {
+            return -1;
+        }
+    }
+    void use_global_mass(double m) {
+        lf2mi_or_die(lf2mi_parts_set_mass(h_, m), "projection mass");
+        lf2mi_or_die(lf2mi_parts_mass_from(h_, LF2MI_MASS_INFO, -1, nullptr), "projection mass");
     }
 
     lf2mi_parts h_;
@@ -493,8 +512,7 @@ inline void projection_init(Field<Real> *f) {
 template <typename part, typename part_info, typename part_dataType>
 void scalarProjectionCIC_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *rho, size_t *oset = NULL,
                                  int flag_where = FROM_INFO) {
-    (void)oset;
-    (void)flag_where;
+    parts->select_mass(oset, flag_where);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_SCALAR, rho->device_ptr(), rho->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "scalarProjectionCIC_proj");
@@ -507,8 +525,7 @@ inline void scalarProjectionCIC_comm(Field<Real> *rho) {
 template <typename part, typename part_info, typename part_dataType>
 void vectorProjectionCIC_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *B, size_t *oset = NULL,
                                  int flag_where = FROM_INFO) {
-    (void)oset;
-    (void)flag_where;
+    parts->select_mass(oset, flag_where);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECTOR, B->device_ptr(), B->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "vectorProjectionCIC_project");
@@ -521,8 +538,7 @@ inline void vectorProjectionCIC_comm(Field<Real> *B) {
 template <typename part, typename part_info, typename part_dataType>
 void VecVecProjectionCIC_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *T, size_t *oset = NULL,
                                  int flag_where = FROM_INFO) {
-    (void)oset;
-    (void)flag_where;
+    parts->select_mass(oset, flag_where);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECVEC, T->device_ptr(), T->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "VecVecProjectionCIC_project");

@@ -187,6 +187,46 @@ int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_
                              double *pos, double *vel, double *mass, int64_t *n_out,
                              void *stream);
 
+/* ---- user particle structs (Particles<part, part_info, part_dataType> with any part
+ * type, particles/LATfield2_Particles.hpp:131-344, :358-428).  The reference keeps the
+ * whole struct per particle and reads fields by byte offset (the mass of
+ * projections.hpp:108-126, :144-152).  A container given a payload of `bytes` per
+ * particle (a multiple of 8, at most 4096; before the first upload) keeps each particle's
+ * struct on the device and moves it with the particle through every re-bucketing.
+ * Particles made by the synthetic initial conditions get a zero payload. */
+int lf2mi_parts_set_payload(lf2mi_parts p, int bytes);
+
+/* addParticle_global from n structs of `stride` bytes at host address `base`: pos[3] and
+ * vel[3] of type Real (real_dtype LF2MI_F32 or LF2MI_F64) at byte offsets off_pos and
+ * off_vel, the ID (int64) at off_id and a per-particle mass (double) at off_mass, each
+ * -1 when absent (IDs then count from 0; masses need a container with masses or a
+ * payload).  With a payload, stride must equal it and the structs are kept whole.
+ * Same filtering and replacement as lf2mi_parts_upload.  Syncs. */
+int lf2mi_parts_upload_aos(lf2mi_parts p, int64_t n, const void *base, int64_t stride, int real_dtype,
+                           int64_t off_pos, int64_t off_vel, int64_t off_id, int64_t off_mass,
+                           int64_t *kept, void *stream);
+
+/* The particles back into n_room structs of `stride` bytes (container order): the payload
+ * first (when kept), then pos and vel (Real of real_dtype, positions as in
+ * lf2mi_parts_download) and the ID at their offsets (-1: not written).  *n_out = count
+ * (LF2MI_ERANGE when it exceeds n_room).  Syncs. */
+int lf2mi_parts_download_aos(lf2mi_parts p, void *base, int64_t n_room, int64_t stride, int real_dtype,
+                             int64_t off_pos, int64_t off_vel, int64_t off_id, int64_t *n_out,
+                             void *stream);
+
+/* The mass the deposits use (the projections' flag_where / oset, projections.hpp:110-126):
+ *   LF2MI_MASS_INFO  the global mass of lf2mi_parts_set_mass (FROM_INFO);
+ *   LF2MI_MASS_PART  per particle: the double at byte `offset` of its payload
+ *                    (FROM_PART), gathered on the device into the per-particle masses;
+ *                    offset -1 uses those masses as they stand (uploaded, or gathered
+ *                    by an earlier call).
+ * The choice holds for later deposits and PM steps (whose mean density follows it).
+ * A container created with LF2MI_PARTS_MASS starts on LF2MI_MASS_PART, others on
+ * LF2MI_MASS_INFO.  Syncs. */
+#define LF2MI_MASS_INFO 0
+#define LF2MI_MASS_PART 1
+int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream);
+
 /* per-cell particle counts into a device int32 array of N^3 entries (bit-exact with
  * the histogram of Particles::getPartCoord keys) */
 int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream);

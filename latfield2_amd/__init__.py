@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64,
+from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART,
                    FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_VECTOR,
                    PROJ_VECVEC, Lf2miError, call, load)
 
@@ -270,6 +270,48 @@ class Particles:
              ctypes.byref(n), _stream())
         k = n.value
         return {"pos": pos[:k], "vel": vel[:k], "mass": mass[:k]}
+
+    # ---- user structs (Particles<part, ...> with any part type; the mass by byte offset of
+    # projections.hpp:108-126)
+    def set_payload(self, nbytes: int):
+        """keep each particle's whole struct (nbytes, a multiple of 8) on the device"""
+        call("lf2mi_parts_set_payload", self.h, int(nbytes))
+
+    @staticmethod
+    def _fields(dtype):
+        names = dtype.names or ()
+        def off(name):
+            return dtype.fields[name][1] if name in names else -1
+        real = dtype.fields["pos"][0].base
+        code = F32 if real == np.float32 else F64
+        return code, off("pos"), off("vel"), off("ID"), off("mass")
+
+    def upload_structs(self, parts) -> int:
+        """addParticle_global from a numpy structured array with fields pos (3 Real), vel
+        (3 Real), optional ID (int64) and mass (float64); returns the particles kept"""
+        a = np.ascontiguousarray(parts)
+        code, op, ov, oi, om = self._fields(a.dtype)
+        kept = ctypes.c_int64(0)
+        call("lf2mi_parts_upload_aos", self.h, a.shape[0], a.ctypes.data_as(ctypes.c_void_p), a.dtype.itemsize,
+             code, op, ov, oi, om, ctypes.byref(kept), _stream())
+        return kept.value
+
+    def download_structs(self, dtype):
+        """the particles as a numpy structured array of `dtype` (container order): the kept
+        structs with pos, vel and ID brought up to date"""
+        dtype = np.dtype(dtype)
+        code, op, ov, oi, _ = self._fields(dtype)
+        out = np.zeros(self.count(), dtype=dtype)
+        n = ctypes.c_int64(0)
+        call("lf2mi_parts_download_aos", self.h, out.ctypes.data_as(ctypes.c_void_p), out.shape[0], dtype.itemsize,
+             code, op, ov, oi, ctypes.byref(n), _stream())
+        return out[:n.value]
+
+    def mass_from(self, where: int, offset: int = -1):
+        """mass of later deposits: MASS_INFO (global) or MASS_PART (the double at byte offset
+        of each kept struct, gathered into the per-particle masses; -1: those masses as they
+        stand)"""
+        call("lf2mi_parts_mass_from", self.h, int(where), int(offset), _stream())
 
     def cell_counts(self):
         torch = _torch()

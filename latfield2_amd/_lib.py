@@ -19,6 +19,7 @@ PROJ_SCALAR, PROJ_VECTOR, PROJ_VECVEC = 0, 1, 2
 DEP_UNFOLDED, DEP_PERIODIC = 0, 1
 ACC_NATIVE, ACC_FIXED64 = 0, 1
 PARTS_ID, PARTS_MASS = 1, 2
+MASS_INFO, MASS_PART = 0, 1
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -61,6 +62,10 @@ SIGNATURES = {
     "lf2mi_parts_download": [_vp, _dp, _dp, _dp, _i64p, _i32p, _vp],
     "lf2mi_parts_cell_counts": [_vp, _vp, _vp],
     "lf2mi_parts_download_box": [_vp, _i32p, _i32p, _i64, _dp, _dp, _dp, _i64p, _vp],
+    "lf2mi_parts_set_payload": [_vp, _i],
+    "lf2mi_parts_upload_aos": [_vp, _i64, _vp, _i64, _i, _i64, _i64, _i64, _i64, _i64p, _vp],
+    "lf2mi_parts_download_aos": [_vp, _vp, _i64, _i64, _i, _i64, _i64, _i64, _i64p, _vp],
+    "lf2mi_parts_mass_from": [_vp, _i, _i64, _vp],
     "lf2mi_parts_ic_lattice": [_vp, _i, _d, _d, _u64, _vp],
     "lf2mi_parts_ic_uniform": [_vp, _i64, _d, _u64, _vp],
     "lf2mi_parts_ic_blobs": [_vp, _i64, _i, _d, _d, _d, _d, _u64, _vp],

@@ -121,6 +121,7 @@ struct Soa32 {
     float *vel[3];
     float *mass;   // may be null
     int64_t *id;   // may be null
+    unsigned long long *pl;  // payload: plw 8-byte words per particle (user structs), may be null
 };
 
 // ---------------------------------------------------------------- binning (FP64 -> cell, offset)
@@ -144,8 +145,9 @@ __device__ __forceinline__ bool bin_one(const double p[3], int N, int &cell, flo
 
 // staged FP64 input: AoS pos/vel (host layout), optional mass/id
 __global__ void bin_upload_kernel(int64_t np, const double *__restrict__ pos, const double *__restrict__ vel,
-                                  const double *__restrict__ mass, const int64_t *__restrict__ id, Soa32 dst,
-                                  Geo g, int *__restrict__ counts) {
+                                  const double *__restrict__ mass, const int64_t *__restrict__ id,
+                                  const unsigned long long *__restrict__ pl, int plw, Soa32 dst, Geo g,
+                                  int *__restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < np;
     int cell = -1;
@@ -158,8 +160,9 @@ __global__ void bin_upload_kernel(int64_t np, const double *__restrict__ pos, co
             dst.off[d][i] = off[d];
             dst.vel[d][i] = (float)vel[3 * i + d];
         }
-        if (dst.mass) dst.mass[i] = (float)mass[i];
+        if (dst.mass) dst.mass[i] = mass ? (float)mass[i] : 1.0f;
         if (dst.id) dst.id[i] = id ? id[i] : i;
+        for (int w = 0; w < plw; ++w) dst.pl[i * plw + w] = pl[i * plw + w];
     }
     const bool act = in && cell >= 0;
     agg_inc(counts, act ? bucket_of(cell, g) : 0, act);
@@ -231,6 +234,7 @@ struct Soa64 {
     double *vel[3];
     double *mass;
     int64_t *id;
+    unsigned long long *pl;  // payload, as in Soa32
 };
 
 __global__ void ic_f64_kernel(IcArgs A, Soa64 dst, int N) {
@@ -261,7 +265,7 @@ __global__ void upload_f64_kernel(int64_t np, const double *__restrict__ pos, co
         c[d] = (int)floor(p * (double)N) % N;
     }
     dst.cell[i] = c[0] + N * (c[1] + N * c[2]);
-    if (dst.mass) dst.mass[i] = mass[i];
+    if (dst.mass) dst.mass[i] = mass ? mass[i] : 1.0;
     if (dst.id) dst.id[i] = id ? id[i] : i;
 }
 
@@ -407,7 +411,8 @@ static constexpr int SCAT_PER = 4;
 static constexpr int SCAT_HT = 2048;
 
 __global__ __launch_bounds__(256) void scatter_kernel(int64_t np, Soa32 src, Soa32 dst, Geo g,
-                                                      const int *__restrict__ start, int *__restrict__ cursor) {
+                                                      const int *__restrict__ start, int *__restrict__ cursor,
+                                                      int plw) {
     __shared__ int hkey[SCAT_HT];
     __shared__ int hval[SCAT_HT];  // group counts, then the bucket's reserved base
     for (int i = threadIdx.x; i < SCAT_HT; i += 256) {
@@ -485,6 +490,8 @@ __global__ __launch_bounds__(256) void scatter_kernel(int64_t np, Soa32 src, Soa
         }
         if (src.mass) dst.mass[j] = m[k];
         if (src.id) dst.id[j] = id[k];
+        const int64_t i = i0 + k * 256;
+        for (int w = 0; w < plw; ++w) dst.pl[j * plw + w] = src.pl[i * plw + w];
     }
 }
 
@@ -1732,6 +1739,8 @@ struct lf2mi_parts_s {
     unsigned long long *vmax_f64;
     std::vector<void *> allocs;
     int64_t bytes;
+    int plw;       // payload words (8 bytes) per particle, 0 without a payload
+    int mass_src;  // deposits use: 0 the global mass, 1 the per-particle masses
 };
 
 namespace lf2mi {
@@ -1779,7 +1788,7 @@ static int sort_into_other(lf2mi_parts_s *p, int src, int64_t nsrc, hipStream_t 
     if (int rc = fill_int(p->bcount, p->g.nbk, 0, st)) return rc;
     LF2MI_PROF("rebucket_scatter", st);
     scatter_kernel<<<grid1(nsrc, 256 * SCAT_PER), 256, 0, st>>>(nsrc, p->b32[src], p->b32[1 - src], p->g, p->bstart,
-                                                                  p->bcount);
+                                                                  p->bcount, p->plw);
     LF2MI_LAUNCHED();
     p->cur = 1 - src;
     return LF2MI_OK;
@@ -1820,21 +1829,58 @@ static int ic_common(lf2mi_parts_s *p, IcArgs &A, hipStream_t st) {
         const int src = 1 - p->cur;
         ic_f32_kernel<<<grid1(A.np), 256, 0, st>>>(A, p->b32[src], p->g, p->bcount);
         LF2MI_LAUNCHED();
+        if (p->plw && A.np > 0)
+            LF2MI_HIP(hipMemsetAsync(p->b32[src].pl, 0, (size_t)A.np * p->plw * sizeof(unsigned long long), st));
         if (int rc = sort_into_other(p, src, A.np, st)) return rc;
         if (int rc = read_total(p, st, &p->np)) return rc;
     } else {
         ic_f64_kernel<<<grid1(A.np), 256, 0, st>>>(A, p->b64, p->N);
         LF2MI_LAUNCHED();
+        if (p->plw && A.np > 0)
+            LF2MI_HIP(hipMemsetAsync(p->b64.pl, 0, (size_t)A.np * p->plw * sizeof(unsigned long long), st));
         LF2MI_HIP(hipStreamSynchronize(st));
         p->np = A.np;
     }
-    p->total_mass = p->mass_global * (double)p->np;  // synthetic particles carry unit mass ratio
+    // synthetic particles: per-particle masses (when kept) are 1
+    p->total_mass = (p->mass_src ? 1.0 : p->mass_global) * (double)p->np;
     return LF2MI_OK;
 }
 
 }  // namespace lf2mi
 
 using namespace lf2mi;
+
+namespace lf2mi {
+
+// sum of the stored per-particle masses into *total
+template <typename T>
+__global__ void sum_mass_kernel(int64_t np, const T *__restrict__ mass, double *__restrict__ total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = i < np ? (double)mass[i] : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
+    if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
+}
+
+// per-particle masses from the double at byte `off` of each payload; sum into *total
+template <typename T>
+__global__ void payload_mass_kernel(int64_t np, const unsigned long long *__restrict__ pl, int plw, int64_t off,
+                                    T *__restrict__ mass, double *__restrict__ total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (i < np) {
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(pl + i * plw);
+        double v;
+        memcpy(&v, b + off, sizeof(double));
+        mass[i] = (T)v;
+        m = (double)(T)v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
+    if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
+}
+
+}  // namespace lf2mi
 
 extern "C" {
 
@@ -1854,6 +1900,8 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
     p->np = 0;
     p->mass_global = 1.0;
     p->cur = 0;
+    p->plw = 0;
+    p->mass_src = (flags & LF2MI_PARTS_MASS) ? 1 : 0;
     int B = 1;
     for (int b = MAXB; b >= 1; --b)
         if (N % b == 0) { B = b; break; }
@@ -1878,6 +1926,7 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
             }
             s.mass = nullptr;
             s.id = nullptr;
+            s.pl = nullptr;
             if (flags & LF2MI_PARTS_MASS) A((void **)&s.mass, cap * sizeof(float));
             if (flags & LF2MI_PARTS_ID) A((void **)&s.id, cap * sizeof(int64_t));
         }
@@ -1899,6 +1948,7 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         }
         s.mass = nullptr;
         s.id = nullptr;
+        s.pl = nullptr;
         if (flags & LF2MI_PARTS_MASS) A((void **)&s.mass, cap * sizeof(double));
         if (flags & LF2MI_PARTS_ID) A((void **)&s.id, cap * sizeof(int64_t));
         A((void **)&p->vmax_f64, sizeof(unsigned long long));
@@ -1944,8 +1994,10 @@ int lf2mi_parts_bytes(lf2mi_parts p, int64_t *bytes) {
     return LF2MI_OK;
 }
 
-int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const double *vel, const double *mass,
-                       const int64_t *id, int64_t *kept, void *stream) {
+// the upload; pl (may be null) holds each input particle's payload (p->plw words, in input
+// order), zeros when the container keeps payloads and none is given
+static int upload_impl(lf2mi_parts p, int64_t np, const double *pos, const double *vel, const double *mass,
+                       const int64_t *id, const unsigned char *pl, int64_t *kept, void *stream) {
     if (!p || np < 0 || (np > 0 && (!pos || !vel))) return fail(LF2MI_EINVAL, "lf2mi_parts_upload: bad argument");
     if (np > p->cap) return fail(LF2MI_ERANGE, "%lld particles exceed the capacity %lld", (long long)np, (long long)p->cap);
     if ((p->flags & LF2MI_PARTS_MASS) && np > 0 && !mass) return fail(LF2MI_EINVAL, "per-particle masses required");
@@ -1976,8 +2028,14 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
         }
         hid[(size_t)k] = id ? id[i] : i;
     }
+    const size_t plb = (size_t)p->plw * sizeof(unsigned long long);
+    std::vector<unsigned char> hpl(plb * (size_t)nk, 0);
+    if (pl)
+        for (int64_t k = 0; k < nk; ++k) memcpy(hpl.data() + plb * (size_t)k, pl + plb * (size_t)keep[(size_t)k], plb);
     double *dpos = nullptr, *dvel = nullptr, *dmass = nullptr;
     int64_t *did = nullptr;
+    unsigned long long *dpl = nullptr;
+    if (plb) LF2MI_HIP(hipMalloc((void **)&dpl, plb * (size_t)std::max<int64_t>(nk, 1)));
     const size_t n3 = (size_t)std::max<int64_t>(nk, 1) * 3;
     LF2MI_HIP(hipMalloc((void **)&dpos, n3 * sizeof(double)));
     LF2MI_HIP(hipMalloc((void **)&dvel, n3 * sizeof(double)));
@@ -1990,6 +2048,7 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
         if (e == hipSuccess) e = hipMemcpyAsync(dvel, hv.data(), nk * 3 * sizeof(double), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(did, hid.data(), nk * sizeof(int64_t), hipMemcpyHostToDevice, st);
         if (e == hipSuccess && mass) e = hipMemcpyAsync(dmass, hm.data(), nk * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && plb) e = hipMemcpyAsync(dpl, hpl.data(), plb * (size_t)nk, hipMemcpyHostToDevice, st);
     }
     if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(upload)", __FILE__, __LINE__);
     if (!rc) {
@@ -1997,7 +2056,8 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
             rc = fill_int(p->bcount, p->g.nbk, 0, st);
             const int src = 1 - p->cur;
             if (!rc && nk > 0) {
-                bin_upload_kernel<<<grid1(nk), 256, 0, st>>>(nk, dpos, dvel, dmass, did, p->b32[src], p->g, p->bcount);
+                bin_upload_kernel<<<grid1(nk), 256, 0, st>>>(nk, dpos, dvel, dmass, did, dpl, p->plw, p->b32[src], p->g,
+                                                             p->bcount);
                 if ((e = hipGetLastError()) != hipSuccess) rc = hip_fail(e, "bin_upload_kernel", __FILE__, __LINE__);
             }
             if (!rc) rc = sort_into_other(p, src, nk, st);
@@ -2006,6 +2066,9 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
             if (nk > 0) {
                 upload_f64_kernel<<<grid1(nk), 256, 0, st>>>(nk, dpos, dvel, dmass, did, p->b64, p->N);
                 if ((e = hipGetLastError()) != hipSuccess) rc = hip_fail(e, "upload_f64_kernel", __FILE__, __LINE__);
+                if (!rc && plb && (e = hipMemcpyAsync(p->b64.pl, dpl, plb * (size_t)nk, hipMemcpyDeviceToDevice, st)) !=
+                                      hipSuccess)
+                    rc = hip_fail(e, "payload", __FILE__, __LINE__);
             }
             if (!rc && (e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_fail(e, "sync", __FILE__, __LINE__);
             p->np = nk;
@@ -2016,10 +2079,16 @@ int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const doubl
     hipFree(dvel);
     hipFree(did);
     if (dmass) hipFree(dmass);
+    if (dpl) hipFree(dpl);
     if (rc) return rc;
-    p->total_mass = mass ? msum : p->mass_global * (double)p->np;
+    p->total_mass = mass && p->mass_src ? msum : p->mass_global * (double)p->np;
     if (kept) *kept = p->np;
     return LF2MI_OK;
+}
+
+int lf2mi_parts_upload(lf2mi_parts p, int64_t np, const double *pos, const double *vel, const double *mass,
+                       const int64_t *id, int64_t *kept, void *stream) {
+    return upload_impl(p, np, pos, vel, mass, id, nullptr, kept, stream);
 }
 
 int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, int64_t *id, int32_t *cell,
@@ -2034,7 +2103,8 @@ int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, 
     LF2MI_HIP(hipMalloc((void **)&dmass, np * sizeof(double)));
     int rc = LF2MI_OK;
     if (p->dtype == LF2MI_F32) {
-        const Soa32 &s = p->b32[p->cur];
+        Soa32 s = p->b32[p->cur];
+        if (!p->mass_src) s.mass = nullptr;  // the global mass is the one in use
         download_f32_kernel<<<grid1(np), 256, 0, st>>>(np, s, p->N, dpos, dvel, dmass, p->mass_global);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && pos) e = hipMemcpyAsync(pos, dpos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
@@ -2053,7 +2123,7 @@ int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, 
         if (e == hipSuccess && pos) e = hipMemcpyAsync(pos, dpos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, dvel, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess && mass) {
-            if (s.mass) e = hipMemcpyAsync(mass, s.mass, np * sizeof(double), hipMemcpyDeviceToHost, st);
+            if (s.mass && p->mass_src) e = hipMemcpyAsync(mass, s.mass, np * sizeof(double), hipMemcpyDeviceToHost, st);
             else for (int64_t i = 0; i < np; ++i) mass[i] = p->mass_global;
         }
         if (e == hipSuccess && id) {
@@ -2068,6 +2138,166 @@ int lf2mi_parts_download(lf2mi_parts p, double *pos, double *vel, double *mass, 
     hipFree(dvel);
     hipFree(dmass);
     return rc;
+}
+
+// ---------------------------------------------------------------- user structs
+int lf2mi_parts_set_payload(lf2mi_parts p, int bytes) {
+    if (!p) return fail(LF2MI_EINVAL, "null container");
+    if (bytes < 0 || bytes > 4096 || bytes % 8) return fail(LF2MI_EINVAL, "payload of %d bytes: a multiple of 8 up to 4096", bytes);
+    if (p->np > 0 || p->plw) return fail(LF2MI_EINVAL, "the payload is set once, before the first upload");
+    if (bytes == 0) return LF2MI_OK;
+    const size_t cap = (size_t)std::max<int64_t>(p->cap, 1);
+    int rc = LF2MI_OK;
+    if (p->dtype == LF2MI_F32) {
+        for (int k = 0; k < 2 && !rc; ++k) {
+            Soa32 &s = p->b32[k];
+            rc = dalloc(p, (void **)&s.pl, cap * (size_t)bytes);
+            if (!rc && !s.mass) rc = dalloc(p, (void **)&s.mass, cap * sizeof(float));
+        }
+    } else {
+        rc = dalloc(p, (void **)&p->b64.pl, cap * (size_t)bytes);
+        if (!rc && !p->b64.mass) rc = dalloc(p, (void **)&p->b64.mass, cap * sizeof(double));
+    }
+    if (rc) return rc;
+    p->plw = bytes / 8;
+    return LF2MI_OK;
+}
+
+static bool real_field_ok(int64_t off, int64_t size, int64_t stride) { return off < 0 || off + size <= stride; }
+
+int lf2mi_parts_upload_aos(lf2mi_parts p, int64_t n, const void *base, int64_t stride, int real_dtype,
+                           int64_t off_pos, int64_t off_vel, int64_t off_id, int64_t off_mass, int64_t *kept,
+                           void *stream) {
+    if (!p || n < 0 || (n > 0 && !base) || stride <= 0) return fail(LF2MI_EINVAL, "lf2mi_parts_upload_aos: bad argument");
+    if (real_dtype != LF2MI_F32 && real_dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad real dtype %d", real_dtype);
+    const int64_t rs = real_dtype == LF2MI_F32 ? 4 : 8;
+    if (off_pos < 0 || off_vel < 0 || !real_field_ok(off_pos, 3 * rs, stride) || !real_field_ok(off_vel, 3 * rs, stride) ||
+        !real_field_ok(off_id, 8, stride) || !real_field_ok(off_mass, 8, stride))
+        return fail(LF2MI_EINVAL, "field offsets outside the %lld-byte struct", (long long)stride);
+    if (p->plw && stride != (int64_t)p->plw * 8)
+        return fail(LF2MI_EINVAL, "struct of %lld bytes against a payload of %d", (long long)stride, p->plw * 8);
+    if (off_mass >= 0 && !(p->dtype == LF2MI_F32 ? (void *)p->b32[0].mass : (void *)p->b64.mass))
+        return fail(LF2MI_EINVAL, "per-particle masses need a container with masses or a payload");
+    const unsigned char *b = static_cast<const unsigned char *>(base);
+    std::vector<double> pos((size_t)n * 3), vel((size_t)n * 3), mass(off_mass >= 0 ? (size_t)n : 0);
+    std::vector<int64_t> id(off_id >= 0 ? (size_t)n : 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const unsigned char *q = b + i * stride;
+        for (int d = 0; d < 3; ++d) {
+            if (rs == 4) {
+                float x, v;
+                memcpy(&x, q + off_pos + 4 * d, 4);
+                memcpy(&v, q + off_vel + 4 * d, 4);
+                pos[3 * i + d] = x;
+                vel[3 * i + d] = v;
+            } else {
+                memcpy(&pos[3 * i + d], q + off_pos + 8 * d, 8);
+                memcpy(&vel[3 * i + d], q + off_vel + 8 * d, 8);
+            }
+        }
+        if (off_id >= 0) memcpy(&id[(size_t)i], q + off_id, 8);
+        if (off_mass >= 0) memcpy(&mass[(size_t)i], q + off_mass, 8);
+    }
+    return upload_impl(p, n, pos.data(), vel.data(), off_mass >= 0 ? mass.data() : nullptr,
+                       off_id >= 0 ? id.data() : nullptr, p->plw ? b : nullptr, kept, stream);
+}
+
+int lf2mi_parts_download_aos(lf2mi_parts p, void *base, int64_t n_room, int64_t stride, int real_dtype,
+                             int64_t off_pos, int64_t off_vel, int64_t off_id, int64_t *n_out, void *stream) {
+    if (!p || stride <= 0 || n_room < 0) return fail(LF2MI_EINVAL, "lf2mi_parts_download_aos: bad argument");
+    if (real_dtype != LF2MI_F32 && real_dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad real dtype %d", real_dtype);
+    const int64_t rs = real_dtype == LF2MI_F32 ? 4 : 8;
+    if (!real_field_ok(off_pos, 3 * rs, stride) || !real_field_ok(off_vel, 3 * rs, stride) || !real_field_ok(off_id, 8, stride))
+        return fail(LF2MI_EINVAL, "field offsets outside the %lld-byte struct", (long long)stride);
+    if (p->plw && stride != (int64_t)p->plw * 8)
+        return fail(LF2MI_EINVAL, "struct of %lld bytes against a payload of %d", (long long)stride, p->plw * 8);
+    const int64_t np = p->np;
+    if (n_out) *n_out = np;
+    if (np > n_room) return fail(LF2MI_ERANGE, "%lld particles exceed the room for %lld", (long long)np, (long long)n_room);
+    if (np == 0) return LF2MI_OK;
+    if (!base) return fail(LF2MI_EINVAL, "lf2mi_parts_download_aos: null buffer");
+    hipStream_t st = as_stream(stream);
+    std::vector<double> pos((size_t)np * 3), vel((size_t)np * 3);
+    std::vector<int64_t> id(off_id >= 0 ? (size_t)np : 0);
+    if (int rc = lf2mi_parts_download(p, pos.data(), vel.data(), nullptr, off_id >= 0 ? id.data() : nullptr, nullptr,
+                                      stream))
+        return rc;
+    unsigned char *b = static_cast<unsigned char *>(base);
+    if (p->plw) {
+        const unsigned long long *pl = p->dtype == LF2MI_F32 ? p->b32[p->cur].pl : p->b64.pl;
+        LF2MI_HIP(hipMemcpyAsync(b, pl, (size_t)np * p->plw * 8, hipMemcpyDeviceToHost, st));
+        LF2MI_HIP(hipStreamSynchronize(st));
+    }
+    for (int64_t i = 0; i < np; ++i) {
+        unsigned char *q = b + i * stride;
+        for (int d = 0; d < 3; ++d) {
+            if (off_pos >= 0) {
+                if (rs == 4) {
+                    const float x = (float)pos[3 * i + d];
+                    memcpy(q + off_pos + 4 * d, &x, 4);
+                } else {
+                    memcpy(q + off_pos + 8 * d, &pos[3 * i + d], 8);
+                }
+            }
+            if (off_vel >= 0) {
+                if (rs == 4) {
+                    const float v = (float)vel[3 * i + d];
+                    memcpy(q + off_vel + 4 * d, &v, 4);
+                } else {
+                    memcpy(q + off_vel + 8 * d, &vel[3 * i + d], 8);
+                }
+            }
+        }
+        if (off_id >= 0) memcpy(q + off_id, &id[(size_t)i], 8);
+    }
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream) {
+    if (!p) return fail(LF2MI_EINVAL, "null container");
+    if (where != LF2MI_MASS_INFO && where != LF2MI_MASS_PART) return fail(LF2MI_EINVAL, "bad mass source %d", where);
+    if (where == LF2MI_MASS_INFO) {
+        p->mass_src = 0;
+        p->total_mass = p->mass_global * (double)p->np;
+        return LF2MI_OK;
+    }
+    const bool has_mass = p->dtype == LF2MI_F32 ? p->b32[0].mass != nullptr : p->b64.mass != nullptr;
+    if (!has_mass) return fail(LF2MI_EINVAL, "per-particle masses need a container with masses or a payload");
+    if (offset >= 0 && (!p->plw || offset + 8 > (int64_t)p->plw * 8)) {
+        return fail(LF2MI_EINVAL, "mass offset %lld outside the payload", (long long)offset);
+    }
+    hipStream_t st = as_stream(stream);
+    double *dsum = nullptr;
+    LF2MI_HIP(hipMalloc((void **)&dsum, sizeof(double)));
+    LF2MI_HIP(hipMemsetAsync(dsum, 0, sizeof(double), st));
+    int rc = LF2MI_OK;
+    if (p->np > 0) {
+        if (offset >= 0) {
+            if (p->dtype == LF2MI_F32)
+                payload_mass_kernel<float><<<grid1(p->np), 256, 0, st>>>(p->np, p->b32[p->cur].pl, p->plw, offset,
+                                                                          p->b32[p->cur].mass, dsum);
+            else
+                payload_mass_kernel<double><<<grid1(p->np), 256, 0, st>>>(p->np, p->b64.pl, p->plw, offset, p->b64.mass,
+                                                                           dsum);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = hip_fail(e, "payload_mass_kernel", __FILE__, __LINE__);
+        } else if (p->dtype == LF2MI_F32) {
+            sum_mass_kernel<float><<<grid1(p->np), 256, 0, st>>>(p->np, p->b32[p->cur].mass, dsum);
+        } else {
+            sum_mass_kernel<double><<<grid1(p->np), 256, 0, st>>>(p->np, p->b64.mass, dsum);
+        }
+    }
+    double total = 0.0;
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(&total, dsum, sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = hip_fail(e, "mass sum", __FILE__, __LINE__);
+    }
+    hipFree(dsum);
+    if (rc) return rc;
+    p->mass_src = 1;
+    p->total_mass = total;
+    return LF2MI_OK;
 }
 
 int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_t max_out, double *pos, double *vel,
@@ -2101,8 +2331,11 @@ int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_
         A.vel = d_vel;
         A.mass = d_mass;
         A.mass_global = p->mass_global;
-        if (p->dtype == LF2MI_F32) download_box_f32_kernel<<<grid1(p->np), 256, 0, st>>>(A, p->b32[p->cur]);
-        else download_box_f64_kernel<<<grid1(p->np), 256, 0, st>>>(A, p->b64);
+        Soa32 s32 = p->b32[p->cur];
+        Soa64 s64 = p->b64;
+        if (!p->mass_src) s32.mass = nullptr, s64.mass = nullptr;
+        if (p->dtype == LF2MI_F32) download_box_f32_kernel<<<grid1(p->np), 256, 0, st>>>(A, s32);
+        else download_box_f64_kernel<<<grid1(p->np), 256, 0, st>>>(A, s64);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_cur, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -2230,7 +2463,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         A.field = (double *)field;
         A.h = halo;
         A.tgt = target;
-        if (p->b64.mass) A.mass_nv = nv;
+        if (p->mass_src && p->b64.mass) A.mass_nv = nv;
+        else A.s.mass = nullptr;
         const unsigned g = grid1(p->np);
         LF2MI_PROF(kind == 0 ? "deposit_f64_scalar" : (kind == 1 ? "deposit_f64_vector" : "deposit_f64_vecvec"), st);
         if (kind == 0) deposit_f64_kernel<0><<<g, 256, 0, st>>>(A);
@@ -2245,7 +2479,10 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     A.istart = p->istart;
     A.item_bucket = p->item_bucket;
     A.g = p->g;
-    A.mass_nv = (float)((p->flags & LF2MI_PARTS_MASS) ? nv : p->mass_global * nv);
+    // per-particle masses only when chosen (lf2mi_parts_mass_from)
+    const bool pm = p->mass_src && A.s.mass;
+    if (!pm) A.s.mass = nullptr;
+    A.mass_nv = (float)(pm ? nv : p->mass_global * nv);
     A.field = (float *)field;
     A.h = halo;
     A.tgt = target;

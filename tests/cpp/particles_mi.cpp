@@ -5,7 +5,11 @@
 //     (rho = m N^3, B_i = m v_i N^3, T_ij = m v_i v_j N^3) in the interior;
 //   - the gathered force of a uniform potential is zero, so updateVel leaves v = 1 and
 //     returns |v| = sqrt(3);
-//   - moveParticles(gpu_drift, dt) moves every particle by dt in each direction.
+//   - moveParticles(gpu_drift, dt) moves every particle by dt in each direction;
+//   - a part type with its own `mass` and another double member: the projection takes the
+//     individual mass by default, any double member with FROM_PART and an offset, and
+//     part_info's mass with FROM_INFO (projections.hpp:110-126); the whole structs come
+//     back from getParticles after a move (projections.hpp:144-152, Particles.hpp:398-427).
 // Exit code = number of failed checks.
 #include <cmath>
 #include <cstdio>
@@ -14,6 +18,24 @@
 #include "LATfield2_mi.hpp"
 
 using namespace LATfield2;
+
+// a user part type: an individual mass and a weight beside the reference's members
+struct part_weighted {
+    long ID;
+    Real pos[3];
+    Real vel[3];
+    double mass;
+    double weight;
+};
+
+// sum of the interior of a scalar field (host copy)
+static double interior_sum(Field<Real> &f, Lattice &lat) {
+    f.toHost();
+    double s = 0.0;
+    Site x(lat);
+    for (x.first(); x.test(); x.next()) s += f(x);
+    return s;
+}
 
 int main(int argc, char **argv) {
     const int N = argc > 1 ? atoi(argv[1]) : 32;
@@ -85,6 +107,62 @@ int main(int argc, char **argv) {
     }
     std::printf("moveParticles: max position error %.3e\n", e_pos);
     if (e_pos > 1e-7) ++fails;
+    // user structs: masses 0.2 + 0.01 (ID mod 5), weights 0.3 + 0.02 (ID mod 3)
+    {
+        Particles<part_weighted, part_simple_info, part_simple_dataType> wp;
+        wp.initialize(info, dtype, &lat_part, box);
+        double msum = 0.0, wsum = 0.0;
+        long k = 0;
+        for (xp.first(); xp.test(); xp.next()) {
+            part_weighted p;
+            p.ID = k;
+            for (int d = 0; d < 3; ++d) {
+                p.pos[d] = ((Real)xp.coord(d) + (Real)0.37) / N;
+                p.vel[d] = 0.5;
+            }
+            p.mass = 0.2 + 0.01 * (k % 5);
+            p.weight = 0.3 + 0.02 * (k % 3);
+            msum += p.mass;
+            wsum += p.weight;
+            wp.addParticle_global(p);
+            ++k;
+        }
+        const double nv = (double)N * N * N;
+        size_t ow = offsetof(part_weighted, weight), om = offsetof(part_simple_info, mass);
+        projection_init(&rho);
+        scalarProjectionCIC_project(&wp, &rho);  // the individual mass
+        scalarProjectionCIC_comm(&rho);
+        const double s_mass = interior_sum(rho, lat) / nv;
+        projection_init(&rho);
+        scalarProjectionCIC_project(&wp, &rho, &ow, FROM_PART);
+        scalarProjectionCIC_comm(&rho);
+        const double s_weight = interior_sum(rho, lat) / nv;
+        projection_init(&rho);
+        scalarProjectionCIC_project(&wp, &rho, &om, FROM_INFO);
+        scalarProjectionCIC_comm(&rho);
+        const double s_info = interior_sum(rho, lat) / nv;
+        const double stol = sizeof(Real) == 8 ? 1e-10 : 2e-4;
+        std::printf("struct masses: individual %.9g (%.9g), FROM_PART weight %.9g (%.9g), FROM_INFO %.9g (%.9g)\n",
+                    s_mass, msum, s_weight, wsum, s_info, 0.1 * k);
+        if (std::fabs(s_mass / msum - 1) > stol || std::fabs(s_weight / wsum - 1) > stol ||
+            std::fabs(s_info / (0.1 * k) - 1) > stol)
+            ++fails;
+        wp.moveParticles(gpu_drift(), 3.0 / N);  // 1.5 cells: many cross a brick face
+        std::vector<part_weighted> back;
+        wp.getParticles(back);
+        long bad = (long)back.size() != k ? 1 : 0;
+        for (const part_weighted &p : back) {
+            if (p.mass != 0.2 + 0.01 * (p.ID % 5) || p.weight != 0.3 + 0.02 * (p.ID % 3)) ++bad;
+            const long c[3] = {p.ID % N, (p.ID / N) % N, p.ID / ((long)N * N)};
+            for (int d = 0; d < 3; ++d) {
+                double w = ((double)c[d] + 0.37) / N + 1.5 / N;
+                if (w >= 1.0) w -= 1.0;
+                if (std::fabs(p.pos[d] - w) > (sizeof(Real) == 8 ? 1e-12 : 1e-6)) ++bad;
+            }
+        }
+        std::printf("struct round trip after a move: %ld mismatches\n", bad);
+        if (bad) ++fails;
+    }
     std::printf("particles_mi N=%d %s: %d failed checks\n", N, sizeof(Real) == 8 ? "double" : "single", fails);
     return fails;
 }
