@@ -548,6 +548,33 @@ inline void VecVecProjectionCIC_comm(Field<Real> *T) {
                                        lf2mi_real_code(), nullptr),
                  "VecVecProjectionCIC_comm");
 }
+// the CIC-NGP hybrids (projections.hpp:405-976)
+template <typename part, typename part_info, typename part_dataType>
+void vectorProjectionCICNGP_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *vel,
+                                    size_t *oset = NULL, int flag_where = FROM_INFO) {
+    parts->select_mass(oset, flag_where);
+    lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECTOR_NGP, vel->device_ptr(), vel->lattice().halo(),
+                               LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
+                 "vectorProjectionCICNGP_project");
+}
+inline void vectorProjectionCICNGP_comm(Field<Real> *vel) {
+    lf2mi_or_die(lf2mi_projection_comm(vel->device_ptr(), vel->lattice().size(0), vel->lattice().halo(),
+                                       LF2MI_PROJ_VECTOR_NGP, lf2mi_real_code(), nullptr),
+                 "vectorProjectionCICNGP_comm");
+}
+template <typename part, typename part_info, typename part_dataType>
+void symtensorProjectionCICNGP_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *Tij,
+                                       size_t *oset = NULL, int flag_where = FROM_INFO) {
+    parts->select_mass(oset, flag_where);
+    lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_SYMTENSOR_NGP, Tij->device_ptr(), Tij->lattice().halo(),
+                               LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
+                 "symtensorProjectionCICNGP_project");
+}
+inline void symtensorProjectionCICNGP_comm(Field<Real> *Tij) {
+    lf2mi_or_die(lf2mi_projection_comm(Tij->device_ptr(), Tij->lattice().size(0), Tij->lattice().halo(),
+                                       LF2MI_PROJ_SYMTENSOR_NGP, lf2mi_real_code(), nullptr),
+                 "symtensorProjectionCICNGP_comm");
+}
 
 }  // namespace LATfield2
 

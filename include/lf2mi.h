@@ -49,6 +49,8 @@ extern "C" {
 #define LF2MI_PROJ_SCALAR 0 /* scalarProjectionCIC_*  (C = 1) */
 #define LF2MI_PROJ_VECTOR 1 /* vectorProjectionCIC_*  (C = 3) */
 #define LF2MI_PROJ_VECVEC 2 /* VecVecProjectionCIC_*  (C = 6) */
+#define LF2MI_PROJ_VECTOR_NGP 3    /* vectorProjectionCICNGP_*    (C = 3) */
+#define LF2MI_PROJ_SYMTENSOR_NGP 4 /* symtensorProjectionCICNGP_* (C = 6) */
 
 /* deposit target */
 #define LF2MI_DEP_UNFOLDED 0 /* like *_project: adds into interior and ghost nodes */
@@ -102,7 +104,10 @@ int lf2mi_ghost_fill(void *field, int N, int halo, int C, int dtype, void *strea
 /* the ghost folds after a deposit, in the reference's order (x, then y, then z):
  *   LF2MI_PROJ_SCALAR: scalarProjectionCIC_comm (particles/projections.hpp:194-300)
  *   LF2MI_PROJ_VECTOR: vectorProjectionCIC_comm (particles/projections.hpp:1192-1351)
- *   LF2MI_PROJ_VECVEC: VecVecProjectionCIC_comm (particles/projections.hpp:1624-1814) */
+ *   LF2MI_PROJ_VECVEC: VecVecProjectionCIC_comm (particles/projections.hpp:1624-1814)
+ *   LF2MI_PROJ_VECTOR_NGP: vectorProjectionCICNGP_comm (particles/projections.hpp:558-681)
+ *   LF2MI_PROJ_SYMTENSOR_NGP: symtensorProjectionCICNGP_comm (particles/projections.hpp:856-976)
+ *     (the two CIC-NGP folds are the scalar fold on each component: upper ghosts only) */
 int lf2mi_projection_comm(void *field, int N, int halo, int kind, int dtype, void *stream);
 
 /* sum over the interior of component `comp` in FP64 (parallel.sum of the driver loop in
@@ -252,6 +257,10 @@ int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha,
  *   kind  LF2MI_PROJ_SCALAR -> scalarProjectionCIC_project  (projections.hpp:76-184)
  *         LF2MI_PROJ_VECTOR -> vectorProjectionCIC_project  (projections.hpp:980-1189)
  *         LF2MI_PROJ_VECVEC -> VecVecProjectionCIC_project  (projections.hpp:1356-1622)
+ *         LF2MI_PROJ_VECTOR_NGP -> vectorProjectionCICNGP_project (projections.hpp:422-549):
+ *             component i nearest-grid-point along axis i, CIC along the other two
+ *         LF2MI_PROJ_SYMTENSOR_NGP -> symtensorProjectionCICNGP_project (projections.hpp:709-847):
+ *             T_ii CIC on the vertices, T_ij (i<j) NGP along i and j, CIC along the third
  *   target LF2MI_DEP_UNFOLDED adds exactly what the reference _project adds (ghost
  *          nodes included, follow with lf2mi_projection_comm); LF2MI_DEP_PERIODIC adds
  *          the folded result into the interior (== _project + _comm on the interior).

@@ -21,14 +21,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART,
-                   FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_VECTOR,
-                   PROJ_VECVEC, Lf2miError, call, load)
+                   FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_SYMTENSOR_NGP,
+                   PROJ_VECTOR, PROJ_VECTOR_NGP, PROJ_VECVEC, Lf2miError, call, load)
 
 __all__ = [
     "Lattice", "Field", "KField", "PlanFFT", "Particles", "projection_init",
     "scalarProjectionCIC_project", "scalarProjectionCIC_comm", "vectorProjectionCIC_project",
     "vectorProjectionCIC_comm", "VecVecProjectionCIC_project", "VecVecProjectionCIC_comm",
-    "poisson_green", "poisson_solve", "pm_step", "FFT_FORWARD", "FFT_BACKWARD", "F32", "F64",
+    "vectorProjectionCICNGP_project", "vectorProjectionCICNGP_comm", "symtensorProjectionCICNGP_project",
+    "symtensorProjectionCICNGP_comm", "poisson_green", "poisson_solve", "pm_step", "FFT_FORWARD", "FFT_BACKWARD", "F32", "F64",
     "Lf2miError", "load",
 ]
 
@@ -368,6 +369,26 @@ def VecVecProjectionCIC_project(parts: Particles, T: Field):
 
 def VecVecProjectionCIC_comm(T: Field):
     call("lf2mi_projection_comm", _ptr(T.t), T.N, T.halo, PROJ_VECVEC, T.code, _stream())
+
+
+def vectorProjectionCICNGP_project(parts: Particles, B: Field):
+    """particles/projections.hpp:422-549: component i nearest-grid-point along axis i"""
+    parts.deposit(PROJ_VECTOR_NGP, B, DEP_UNFOLDED)
+
+
+def vectorProjectionCICNGP_comm(B: Field):
+    """particles/projections.hpp:558-681"""
+    call("lf2mi_projection_comm", _ptr(B.t), B.N, B.halo, PROJ_VECTOR_NGP, B.code, _stream())
+
+
+def symtensorProjectionCICNGP_project(parts: Particles, T: Field):
+    """particles/projections.hpp:709-847: T_ii CIC, T_ij NGP along i and j"""
+    parts.deposit(PROJ_SYMTENSOR_NGP, T, DEP_UNFOLDED)
+
+
+def symtensorProjectionCICNGP_comm(T: Field):
+    """particles/projections.hpp:856-976"""
+    call("lf2mi_projection_comm", _ptr(T.t), T.N, T.halo, PROJ_SYMTENSOR_NGP, T.code, _stream())
 
 
 def pm_step(parts: Particles, plan: PlanFFT, rho: Field, phi: Field, dt_kick: float, dt_drift: float,

@@ -15,7 +15,7 @@ F32 = 0
 F64 = 1
 FFT_FORWARD = 1
 FFT_BACKWARD = -1
-PROJ_SCALAR, PROJ_VECTOR, PROJ_VECVEC = 0, 1, 2
+PROJ_SCALAR, PROJ_VECTOR, PROJ_VECVEC, PROJ_VECTOR_NGP, PROJ_SYMTENSOR_NGP = 0, 1, 2, 3, 4
 DEP_UNFOLDED, DEP_PERIODIC = 0, 1
 ACC_NATIVE, ACC_FIXED64 = 0, 1
 PARTS_ID, PARTS_MASS = 1, 2

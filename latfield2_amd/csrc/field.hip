@@ -347,6 +347,9 @@ int lf2mi_projection_comm(void *field, int N, int halo, int kind, int dtype, voi
         case LF2MI_PROJ_SCALAR: C = 1; down = 0u; klo = 0; break;     // y fold: z in [0, N]
         case LF2MI_PROJ_VECTOR: C = 3; down = 7u; klo = -1; break;    // y fold: z in [-1, N]
         case LF2MI_PROJ_VECVEC: C = 6; down = (1u << 1) | (1u << 2) | (1u << 4); klo = -1; break;
+        // the CIC-NGP folds are the scalar fold on every component
+        case LF2MI_PROJ_VECTOR_NGP: C = 3; down = 0u; klo = 0; break;
+        case LF2MI_PROJ_SYMTENSOR_NGP: C = 6; down = 0u; klo = 0; break;
         default: return fail(LF2MI_EINVAL, "lf2mi_projection_comm: bad kind %d", kind);
     }
     if (int rc = check_field(N, halo, C, dtype)) return rc;

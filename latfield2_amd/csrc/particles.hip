@@ -540,16 +540,20 @@ __device__ __forceinline__ bool item_range(int item, const int *__restrict__ sta
 //      rounding past the FP32 run sums is the quantum 2^-51 of that bound;
 //   C. the node window (all components, component-fastest) is added into the field with
 //      one global atomic per non-zero value (periodic wrap or the unfolded ghost layer).
-// Node offsets per axis: {0, 1} on the vertex grid and {-1, 0, +1} on the link grid
-// (nodes at n + 1/2), so 8 (scalar, T_ii), 12 (vector component) or 18 (T_ij) terms per
-// particle and component; the window spans nodes [LO, B] per axis, LO = -1 when a
-// component of the kind sits on links.
+// Node offsets per axis: {0, 1} on the vertex grid, {-1, 0, +1} on the link grid (nodes at
+// n + 1/2) and {0} on a nearest-grid-point axis, so 8 (scalar, T_ii), 12 (vector component),
+// 18 (T_ij), 4 (CIC-NGP vector component) or 2 (CIC-NGP T_ij) terms per particle and
+// component; the window spans nodes [LO, B] per axis, LO = -1 when a component of the kind
+// sits on links.
 template <int KIND> struct KindC;
 // CW: components the node window holds at a time (VecVec runs two passes of three, which
 // keeps its LDS at the vector kind's)
 template <> struct KindC<0> { static constexpr int C = 1, CW = 1, LO = 0; };
 template <> struct KindC<1> { static constexpr int C = 3, CW = 3, LO = -1; };
 template <> struct KindC<2> { static constexpr int C = 6, CW = 3, LO = -1; };
+// the CIC-NGP hybrids (vertex and nearest-grid-point axes only)
+template <> struct KindC<3> { static constexpr int C = 3, CW = 3, LO = 0; };
+template <> struct KindC<4> { static constexpr int C = 6, CW = 3, LO = 0; };
 
 // link-grid weights of one axis as a 3-vector over node offsets {-1, 0, +1}
 // (projections.hpp:1058-1071: the branch at the link midpoint)
@@ -567,18 +571,38 @@ __device__ __forceinline__ void link3(float u, float &wm, float &w0, float &wp) 
     }
 }
 
-// component c of kind KIND: stagger mask (bit d: link grid along axis d) and velocity factors
+// component c of kind KIND: stagger mask (bit d: link grid along axis d), nearest-grid-point
+// mask (bit d: the single node of the particle's cell along axis d, weight 1) and velocity
+// factors; an axis in neither mask takes the vertex weights
 template <int KIND, int COMP> struct Comp;
-template <> struct Comp<0, 0> { static constexpr int MASK = 0, VA = -1, VB = -1; };
-template <> struct Comp<1, 0> { static constexpr int MASK = 1, VA = 0, VB = -1; };
-template <> struct Comp<1, 1> { static constexpr int MASK = 2, VA = 1, VB = -1; };
-template <> struct Comp<1, 2> { static constexpr int MASK = 4, VA = 2, VB = -1; };
-template <> struct Comp<2, 0> { static constexpr int MASK = 0, VA = 0, VB = 0; };  // (0,0)
-template <> struct Comp<2, 1> { static constexpr int MASK = 3, VA = 0, VB = 1; };  // (0,1)
-template <> struct Comp<2, 2> { static constexpr int MASK = 5, VA = 0, VB = 2; };  // (0,2)
-template <> struct Comp<2, 3> { static constexpr int MASK = 0, VA = 1, VB = 1; };  // (1,1)
-template <> struct Comp<2, 4> { static constexpr int MASK = 6, VA = 1, VB = 2; };  // (1,2)
-template <> struct Comp<2, 5> { static constexpr int MASK = 0, VA = 2, VB = 2; };  // (2,2)
+template <> struct Comp<0, 0> { static constexpr int MASK = 0, NGP = 0, VA = -1, VB = -1; };
+template <> struct Comp<1, 0> { static constexpr int MASK = 1, NGP = 0, VA = 0, VB = -1; };
+template <> struct Comp<1, 1> { static constexpr int MASK = 2, NGP = 0, VA = 1, VB = -1; };
+template <> struct Comp<1, 2> { static constexpr int MASK = 4, NGP = 0, VA = 2, VB = -1; };
+template <> struct Comp<2, 0> { static constexpr int MASK = 0, NGP = 0, VA = 0, VB = 0; };  // (0,0)
+template <> struct Comp<2, 1> { static constexpr int MASK = 3, NGP = 0, VA = 0, VB = 1; };  // (0,1)
+template <> struct Comp<2, 2> { static constexpr int MASK = 5, NGP = 0, VA = 0, VB = 2; };  // (0,2)
+template <> struct Comp<2, 3> { static constexpr int MASK = 0, NGP = 0, VA = 1, VB = 1; };  // (1,1)
+template <> struct Comp<2, 4> { static constexpr int MASK = 6, NGP = 0, VA = 1, VB = 2; };  // (1,2)
+template <> struct Comp<2, 5> { static constexpr int MASK = 0, NGP = 0, VA = 2, VB = 2; };  // (2,2)
+// vectorProjectionCICNGP_project (projections.hpp:422-549): component i is NGP along i
+template <> struct Comp<3, 0> { static constexpr int MASK = 0, NGP = 1, VA = 0, VB = -1; };
+template <> struct Comp<3, 1> { static constexpr int MASK = 0, NGP = 2, VA = 1, VB = -1; };
+template <> struct Comp<3, 2> { static constexpr int MASK = 0, NGP = 4, VA = 2, VB = -1; };
+// symtensorProjectionCICNGP_project (projections.hpp:709-847): T_ii on the vertices, T_ij
+// NGP along i and j and CIC along the third axis
+template <> struct Comp<4, 0> { static constexpr int MASK = 0, NGP = 0, VA = 0, VB = 0; };  // (0,0)
+template <> struct Comp<4, 1> { static constexpr int MASK = 0, NGP = 3, VA = 0, VB = 1; };  // (0,1)
+template <> struct Comp<4, 2> { static constexpr int MASK = 0, NGP = 5, VA = 0, VB = 2; };  // (0,2)
+template <> struct Comp<4, 3> { static constexpr int MASK = 0, NGP = 0, VA = 1, VB = 1; };  // (1,1)
+template <> struct Comp<4, 4> { static constexpr int MASK = 0, NGP = 6, VA = 1, VB = 2; };  // (1,2)
+template <> struct Comp<4, 5> { static constexpr int MASK = 0, NGP = 0, VA = 2, VB = 2; };  // (2,2)
+
+// nodes per axis d of a component: 3 on links, 1 for NGP, 2 on vertices
+template <int MASK, int NGP, int D>
+__host__ __device__ constexpr int axis_nodes() {
+    return (MASK >> D) & 1 ? 3 : ((NGP >> D) & 1 ? 1 : 2);
+}
 
 struct DepArgs {
     Soa32 s;
@@ -663,11 +687,11 @@ __device__ __forceinline__ unsigned long long to_window(float x, double scale) {
 __device__ __forceinline__ unsigned long long to_window(unsigned long long x, double) { return x; }
 
 // add a cell's NS node terms into the LDS node window of one component (edge NWE,
-// origin LO): node offsets per axis start at -1 on link axes, 0 on vertex axes
-template <int MASK, int LO, typename ACC, int NS>
+// origin LO): node offsets per axis start at -1 on link axes, 0 on the others
+template <int MASK, int NGP, int LO, typename ACC, int NS>
 __device__ __forceinline__ void window_add(unsigned long long *nd, int cell, int B, int NWE, const ACC (&acc)[NS],
                                            double scale) {
-    constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
+    constexpr int SX = axis_nodes<MASK, NGP, 0>(), SY = axis_nodes<MASK, NGP, 1>(), SZ = axis_nodes<MASK, NGP, 2>();
     const int lx = cell % B, ly = (cell / B) % B, lz = cell / (B * B);
     const int base = (lx + ((MASK & 1) ? -1 : 0) - LO) + NWE * ((ly + ((MASK & 2) ? -1 : 0) - LO) +
                                                               NWE * (lz + ((MASK & 4) ? -1 : 0) - LO));
@@ -687,9 +711,9 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
                                                   const float4 (&pd)[DEP_PER], const float4 (&pv)[DEP_PER],
                                                   int ckey, bool any_shared, bool carry_end, unsigned long long *nodes,
                                                   double scale) {
-    constexpr int MASK = Comp<KIND, COMP>::MASK;
+    constexpr int MASK = Comp<KIND, COMP>::MASK, NGP = Comp<KIND, COMP>::NGP;
     constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
-    constexpr int SX = (MASK & 1) ? 3 : 2, SY = (MASK & 2) ? 3 : 2, SZ = (MASK & 4) ? 3 : 2;
+    constexpr int SX = axis_nodes<MASK, NGP, 0>(), SY = axis_nodes<MASK, NGP, 1>(), SZ = axis_nodes<MASK, NGP, 2>();
     constexpr int NS = SX * SY * SZ;
     constexpr int LO = KindC<KIND>::LO;
     constexpr int NWMAX = MAXB + 1 - LO;
@@ -704,7 +728,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
         if (c < 0) continue;  // past the end: every later particle of the run is too
         if (c != cur) {
             if (cur >= 0) {
-                window_add<MASK, LO>(nd, cur, B, NWE, acc, scale);
+                window_add<MASK, NGP, LO>(nd, cur, B, NWE, acc, scale);
 #pragma unroll
                 for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
             }
@@ -719,10 +743,13 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
         }
         float wx[3], wy[3], wz[3];
         if constexpr (MASK & 1) link3(d.x, wx[0], wx[1], wx[2]);
+        else if constexpr (NGP & 1) wx[0] = 1.0f;
         else { wx[0] = 1.0f - d.x; wx[1] = d.x; }
         if constexpr (MASK & 2) link3(d.y, wy[0], wy[1], wy[2]);
+        else if constexpr (NGP & 2) wy[0] = 1.0f;
         else { wy[0] = 1.0f - d.y; wy[1] = d.y; }
         if constexpr (MASK & 4) link3(d.z, wz[0], wz[1], wz[2]);
+        else if constexpr (NGP & 4) wz[0] = 1.0f;
         else { wz[0] = 1.0f - d.z; wz[1] = d.z; }
 #pragma unroll
         for (int kz = 0; kz < SZ; ++kz)
@@ -734,7 +761,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
     }
     // the run's last cell: summed over the lanes that share it (uniform branch)
     if (any_shared) seg_scan(ckey, acc);
-    if (carry_end) window_add<MASK, LO>(nd, cur, B, NWE, acc, scale);
+    if (carry_end) window_add<MASK, NGP, LO>(nd, cur, B, NWE, acc, scale);
 }
 
 // item maxima: |q| and |v_d| as the bits of non-negative floats (ordered like the values)
@@ -953,14 +980,14 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
         }
     };
     flush(0);
-    if constexpr (KIND == 2) {
+    if constexpr (C == 6) {
         // the window's second pass: T_11, T_12, T_22
         __syncthreads();
         for (int i = threadIdx.x; i < CW * NW3; i += DEP_WG) nodes[i] = 0ull;
         __syncthreads();
-        LF2MI_DEP_COMP(2, 3)
-        LF2MI_DEP_COMP(2, 4)
-        LF2MI_DEP_COMP(2, 5)
+        LF2MI_DEP_COMP(KIND, 3)
+        LF2MI_DEP_COMP(KIND, 4)
+        LF2MI_DEP_COMP(KIND, 5)
         __syncthreads();
         flush(3);
     }
@@ -1518,15 +1545,17 @@ struct Dep64Args {
     int h, tgt;
 };
 
+// NX, NY, NZ: nodes per axis (1 on a nearest-grid-point axis, whose weight is wx[0] = 1)
+template <int NX = 2, int NY = 2, int NZ = 2>
 __device__ __forceinline__ void add8_64(double *f, int C, int comp, int N, int h, int tgt, int cx, int cy, int cz,
                                         int lox, int loy, int loz, const double wx[2], const double wy[2],
                                         const double wz[2], double q) {
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < NZ; ++k)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < NY; ++j)
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < NX; ++i) {
                 int x = cx + lox + i, y = cy + loy + j, z = cz + loz + k;
                 if (tgt == LF2MI_DEP_PERIODIC) {
                     x = wrapi(x, N);
@@ -1572,6 +1601,19 @@ __global__ void deposit_f64_kernel(Dep64Args A) {
             add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], llo[0], 0, 0, lw[0], vw[1], vw[2], m * v[0]);
             add8_64(A.field, C, 1, N, A.h, A.tgt, c[0], c[1], c[2], 0, llo[1], 0, vw[0], lw[1], vw[2], m * v[1]);
             add8_64(A.field, C, 2, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, llo[2], vw[0], vw[1], lw[2], m * v[2]);
+        } else if constexpr (KIND == 3) {
+            const double one[2] = {1.0, 0.0};
+            add8_64<1, 2, 2>(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, one, vw[1], vw[2], m * v[0]);
+            add8_64<2, 1, 2>(A.field, C, 1, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], one, vw[2], m * v[1]);
+            add8_64<2, 2, 1>(A.field, C, 2, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], one, m * v[2]);
+        } else if constexpr (KIND == 4) {
+            const double one[2] = {1.0, 0.0};
+            add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[0] * v[0]);
+            add8_64(A.field, C, 3, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[1] * v[1]);
+            add8_64(A.field, C, 5, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[2] * v[2]);
+            add8_64<1, 1, 2>(A.field, C, 1, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, one, one, vw[2], m * v[0] * v[1]);
+            add8_64<1, 2, 1>(A.field, C, 2, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, one, vw[1], one, m * v[0] * v[2]);
+            add8_64<2, 1, 1>(A.field, C, 4, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], one, one, m * v[1] * v[2]);
         } else {
             add8_64(A.field, C, 0, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[0] * v[0]);
             add8_64(A.field, C, 3, N, A.h, A.tgt, c[0], c[1], c[2], 0, 0, 0, vw[0], vw[1], vw[2], m * v[1] * v[1]);
@@ -2445,7 +2487,8 @@ int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha, do
 
 int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, void *stream) {
     if (!p || !field) return fail(LF2MI_EINVAL, "lf2mi_deposit: null argument");
-    if (kind < LF2MI_PROJ_SCALAR || kind > LF2MI_PROJ_VECVEC) return fail(LF2MI_EINVAL, "bad projection kind %d", kind);
+    if (kind < LF2MI_PROJ_SCALAR || kind > LF2MI_PROJ_SYMTENSOR_NGP)
+        return fail(LF2MI_EINVAL, "bad projection kind %d", kind);
     if (halo < 1) return fail(LF2MI_EINVAL, "the field has to have at least a halo of 1");
     if (target != LF2MI_DEP_UNFOLDED && target != LF2MI_DEP_PERIODIC) return fail(LF2MI_EINVAL, "bad target %d", target);
     if (acc != LF2MI_ACC_NATIVE && acc != LF2MI_ACC_FIXED64) return fail(LF2MI_EINVAL, "bad accumulation mode %d", acc);
@@ -2466,10 +2509,16 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         if (p->mass_src && p->b64.mass) A.mass_nv = nv;
         else A.s.mass = nullptr;
         const unsigned g = grid1(p->np);
-        LF2MI_PROF(kind == 0 ? "deposit_f64_scalar" : (kind == 1 ? "deposit_f64_vector" : "deposit_f64_vecvec"), st);
-        if (kind == 0) deposit_f64_kernel<0><<<g, 256, 0, st>>>(A);
-        else if (kind == 1) deposit_f64_kernel<1><<<g, 256, 0, st>>>(A);
-        else deposit_f64_kernel<2><<<g, 256, 0, st>>>(A);
+        static const char *const names64[5] = {"deposit_f64_scalar", "deposit_f64_vector", "deposit_f64_vecvec",
+                                               "deposit_f64_vector_ngp", "deposit_f64_symtensor_ngp"};
+        LF2MI_PROF(names64[kind], st);
+        switch (kind) {
+            case 0: deposit_f64_kernel<0><<<g, 256, 0, st>>>(A); break;
+            case 1: deposit_f64_kernel<1><<<g, 256, 0, st>>>(A); break;
+            case 2: deposit_f64_kernel<2><<<g, 256, 0, st>>>(A); break;
+            case 3: deposit_f64_kernel<3><<<g, 256, 0, st>>>(A); break;
+            default: deposit_f64_kernel<4><<<g, 256, 0, st>>>(A); break;
+        }
         LF2MI_LAUNCHED();
         return LF2MI_OK;
     }
@@ -2516,14 +2565,16 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         }
         return LF2MI_OK;
     }
-    LF2MI_PROF(kind == 0 ? "deposit_f32_scalar" : (kind == 1 ? "deposit_f32_vector" : "deposit_f32_vecvec"), st);
+    static const char *const names32[5] = {"deposit_f32_scalar", "deposit_f32_vector", "deposit_f32_vecvec",
+                                           "deposit_f32_vector_ngp", "deposit_f32_symtensor_ngp"};
+    LF2MI_PROF(names32[kind], st);
     const bool b8 = p->g.B == 8;
-    if (kind == 0) {
-        launch_deposit<0, float>(A, b8, g, st);
-    } else if (kind == 1) {
-        launch_deposit<1, float>(A, b8, g, st);
-    } else {
-        launch_deposit<2, float>(A, b8, g, st);
+    switch (kind) {
+        case 0: launch_deposit<0, float>(A, b8, g, st); break;
+        case 1: launch_deposit<1, float>(A, b8, g, st); break;
+        case 2: launch_deposit<2, float>(A, b8, g, st); break;
+        case 3: launch_deposit<3, float>(A, b8, g, st); break;
+        default: launch_deposit<4, float>(A, b8, g, st); break;
     }
     LF2MI_LAUNCHED();
     return LF2MI_OK;

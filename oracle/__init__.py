@@ -56,11 +56,16 @@ def lib():
             "orc_scalar_project": [_dp, _dp, ctypes.c_double, _i64, _dp, ctypes.c_int, ctypes.c_int],
             "orc_vector_project": [_dp, _dp, _dp, ctypes.c_double, _i64, _dp, ctypes.c_int, ctypes.c_int],
             "orc_vecvec_project": [_dp, _dp, _dp, ctypes.c_double, _i64, _dp, ctypes.c_int, ctypes.c_int],
+            "orc_vector_ngp_project": [_dp, _dp, _dp, ctypes.c_double, _i64, _dp, ctypes.c_int, ctypes.c_int],
+            "orc_symtensor_ngp_project": [_dp, _dp, _dp, ctypes.c_double, _i64, _dp, ctypes.c_int,
+                                          ctypes.c_int],
+            "orc_vector_ngp_comm": [_dp, ctypes.c_int, ctypes.c_int],
+            "orc_symtensor_ngp_comm": [_dp, ctypes.c_int, ctypes.c_int],
             "orc_scalar_comm": [_dp, ctypes.c_int, ctypes.c_int],
             "orc_vector_comm": [_dp, ctypes.c_int, ctypes.c_int],
             "orc_vecvec_comm": [_dp, ctypes.c_int, ctypes.c_int],
             "orc_brute_deposit": [_dp, _dp, _dp, ctypes.c_double, _i64, ctypes.c_int, _dp,
-                                  ctypes.c_int, ctypes.c_int, _dp],
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
             "orc_fft_r2c": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "orc_fft_c2r": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "orc_poisson_green": [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -177,6 +182,34 @@ def vecvec_project(pos, vel, T, N, h=1, mass=None, mass_global=1.0):
     return T
 
 
+def vector_ngp_project(pos, vel, B, N, h=1, mass=None, mass_global=1.0):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    m = _mass_arg(mass)
+    _check(lib().orc_vector_ngp_project(_p(pos), _p(vel), _p(m), mass_global, pos.shape[0], _p(B), N, h),
+           "orc_vector_ngp_project")
+    return B
+
+
+def symtensor_ngp_project(pos, vel, T, N, h=1, mass=None, mass_global=1.0):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    m = _mass_arg(mass)
+    _check(lib().orc_symtensor_ngp_project(_p(pos), _p(vel), _p(m), mass_global, pos.shape[0], _p(T), N, h),
+           "orc_symtensor_ngp_project")
+    return T
+
+
+def vector_ngp_comm(B, N, h=1):
+    _check(lib().orc_vector_ngp_comm(_p(B), N, h), "orc_vector_ngp_comm")
+    return B
+
+
+def symtensor_ngp_comm(T, N, h=1):
+    _check(lib().orc_symtensor_ngp_comm(_p(T), N, h), "orc_symtensor_ngp_comm")
+    return T
+
+
 def scalar_comm(rho, N, h=1):
     _check(lib().orc_scalar_comm(_p(rho), N, h), "orc_scalar_comm")
     return rho
@@ -192,15 +225,17 @@ def vecvec_comm(T, N, h=1):
     return T
 
 
-def brute_deposit(pos, N, stagger=(0.0, 0.0, 0.0), vel=None, va=-1, vb=-1, mass=None, mass_global=1.0):
-    """periodic hat-function deposit of one component; returns [z, y, x] interior array"""
+def brute_deposit(pos, N, stagger=(0.0, 0.0, 0.0), vel=None, va=-1, vb=-1, mass=None, mass_global=1.0,
+                  ngp=0):
+    """periodic hat-function deposit of one component (nearest-grid-point along the axes
+    in the bit mask ``ngp``); returns [z, y, x] interior array"""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
     s = np.ascontiguousarray(stagger, dtype=np.float64)
     out = np.zeros((N, N, N), dtype=np.float64)
     m = _mass_arg(mass)
     _check(lib().orc_brute_deposit(_p(pos), _p(v), _p(m), mass_global, pos.shape[0], N, _p(s),
-                                   va, vb, _p(out)), "orc_brute_deposit")
+                                   va, vb, ngp, _p(out)), "orc_brute_deposit")
     return out
 
 

@@ -338,6 +338,135 @@ int orc_vecvec_project(const double *pos, const double *vel, const double *mass,
     return 0;
 }
 
+/* vectorProjectionCICNGP_project (particles/projections.hpp:422-549): per cell, component
+ * i of m v / res^3 goes to the cell's own node along axis i (the link x + e_i/2 that holds
+ * the particle) and to the four vertex nodes {0,1}^2 of the other two axes with the CIC
+ * weights; the reference's vi[12] per-cell sums are kept here too. */
+int orc_vector_ngp_project(const double *pos, const double *vel, const double *mass,
+                           double mass_global, i64 np, double *B, int N, int h) {
+    if (!pos || !vel || !B || np < 0 || N < 1 || h < 1) return ORC_EBADARG;
+    cell_lists L;
+    int rc = build_lists(pos, np, N, &L);
+    if (rc) return rc;
+    const double res = 1.0 / N;
+    const double cicVol = res * res * res;
+    for (int parity = 0; parity < 2; ++parity) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int z = parity; z < N; z += 2) {
+            for (int y = 0; y < N; ++y) {
+                for (int x = 0; x < N; ++x) {
+                    const i64 cell = x + (i64)N * (y + (i64)N * z);
+                    const i64 b = L.start[cell], e = L.start[cell + 1];
+                    if (b == e) continue;
+                    const double ref[3] = {x * res, y * res, z * res};
+                    /* vi[4 i + 2 hi + lo]: component i, corner (lo, hi) on the two other
+                     * axes taken in increasing order */
+                    double vi[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    for (i64 q = b; q < e; ++q) {
+                        const i64 p = L.order[q];
+                        double up[3], dn[3];
+                        for (int d = 0; d < 3; ++d) {
+                            up[d] = (pos[3 * p + d] - ref[d]) / res;
+                            dn[d] = 1.0 - up[d];
+                        }
+                        const double m = pmass(mass, mass_global, p) / cicVol;
+                        for (int i = 0; i < 3; ++i) {
+                            const int a0 = i == 0 ? 1 : 0, a1 = i == 2 ? 1 : 2;
+                            const double mv = m * vel[3 * p + i];
+                            vi[4 * i + 0] += mv * dn[a0] * dn[a1];
+                            vi[4 * i + 1] += mv * up[a0] * dn[a1];
+                            vi[4 * i + 2] += mv * dn[a0] * up[a1];
+                            vi[4 * i + 3] += mv * up[a0] * up[a1];
+                        }
+                    }
+                    for (int i = 0; i < 3; ++i) {
+                        const int a0 = i == 0 ? 1 : 0, a1 = i == 2 ? 1 : 2;
+                        for (int k = 0; k < 4; ++k) {
+                            int o[3] = {x, y, z};
+                            o[a0] += k & 1;
+                            o[a1] += (k >> 1) & 1;
+                            B[sidx(N, h, o[0], o[1], o[2]) * 3 + i] += vi[4 * i + k];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free_lists(&L);
+    return 0;
+}
+
+/* symtensorProjectionCICNGP_project (particles/projections.hpp:709-847): T_ii += m v_i^2
+ * with the eight CIC vertex weights; T_ij (i<j) += m v_i v_j at the cell's node along i and
+ * j, split between the cell and its +1 neighbour along the third axis with the CIC weights.
+ * Symmetric storage (0,0)=0 (0,1)=1 (0,2)=2 (1,1)=3 (1,2)=4 (2,2)=5. */
+int orc_symtensor_ngp_project(const double *pos, const double *vel, const double *mass,
+                              double mass_global, i64 np, double *T, int N, int h) {
+    if (!pos || !vel || !T || np < 0 || N < 1 || h < 1) return ORC_EBADARG;
+    cell_lists L;
+    int rc = build_lists(pos, np, N, &L);
+    if (rc) return rc;
+    const double res = 1.0 / N;
+    const double cicVol = res * res * res;
+    static const int diag[3] = {0, 3, 5};
+    static const int offd[3] = {1, 2, 4};     /* (0,1) (0,2) (1,2) */
+    static const int third[3] = {2, 1, 0};    /* the CIC axis of each off-diagonal entry */
+    static const int va[3] = {0, 0, 1}, vb[3] = {1, 2, 2};
+    for (int parity = 0; parity < 2; ++parity) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int z = parity; z < N; z += 2) {
+            for (int y = 0; y < N; ++y) {
+                for (int x = 0; x < N; ++x) {
+                    const i64 cell = x + (i64)N * (y + (i64)N * z);
+                    const i64 b = L.start[cell], e = L.start[cell + 1];
+                    if (b == e) continue;
+                    const double ref[3] = {x * res, y * res, z * res};
+                    double tii[24], tij[6];
+                    memset(tii, 0, sizeof tii);
+                    memset(tij, 0, sizeof tij);
+                    for (i64 q = b; q < e; ++q) {
+                        const i64 p = L.order[q];
+                        double up[3], dn[3];
+                        for (int d = 0; d < 3; ++d) {
+                            up[d] = (pos[3 * p + d] - ref[d]) / res;
+                            dn[d] = 1.0 - up[d];
+                        }
+                        const double m = pmass(mass, mass_global, p) / cicVol;
+                        const double *u = vel + 3 * p;
+                        for (int i = 0; i < 3; ++i) {
+                            const double mvv = m * u[i] * u[i];
+                            /* corner bits: 4 = +x, 2 = +y, 1 = +z */
+                            for (int cb = 0; cb < 8; ++cb) {
+                                const double wx = (cb & 4) ? up[0] : dn[0];
+                                const double wy = (cb & 2) ? up[1] : dn[1];
+                                const double wz = (cb & 1) ? up[2] : dn[2];
+                                tii[8 * i + cb] += mvv * wx * wy * wz;
+                            }
+                        }
+                        for (int k = 0; k < 3; ++k) {
+                            const double mvv = m * u[va[k]] * u[vb[k]];
+                            tij[2 * k + 0] += mvv * dn[third[k]];
+                            tij[2 * k + 1] += mvv * up[third[k]];
+                        }
+                    }
+                    for (int i = 0; i < 3; ++i)
+                        for (int cb = 0; cb < 8; ++cb)
+                            T[sidx(N, h, x + ((cb >> 2) & 1), y + ((cb >> 1) & 1), z + (cb & 1)) * 6 + diag[i]] +=
+                                tii[8 * i + cb];
+                    for (int k = 0; k < 3; ++k)
+                        for (int s = 0; s < 2; ++s) {
+                            int o[3] = {x, y, z};
+                            o[third[k]] += s;
+                            T[sidx(N, h, o[0], o[1], o[2]) * 6 + offd[k]] += tij[2 * k + s];
+                        }
+                }
+            }
+        }
+    }
+    free_lists(&L);
+    return 0;
+}
+
 /* ---------------------------------------------------------------------------
  * Ghost folds (the *_comm functions) on one process.  Coordinates below are
  * interior coordinates; the reference loops use storage indices with
@@ -400,9 +529,30 @@ int orc_vecvec_comm(double *T, int N, int h) {
     return staggered_comm(T, N, h, 6, down, 3);
 }
 
+/* vectorProjectionCICNGP_comm (particles/projections.hpp:558-681) and
+ * symtensorProjectionCICNGP_comm (:856-976): the ranges of scalarProjectionCIC_comm on
+ * every component (x: y,z in [-1,N]; y: x in [0,N), z in [0,N]; z: x,y in [0,N)). */
+static int upper_comm(double *f, int N, int h, int C) {
+    if (!f || N < 1 || h < 1) return ORC_EBADARG;
+    for (int z = -1; z <= N; ++z)
+        for (int y = -1; y <= N; ++y)
+            for (int c = 0; c < C; ++c) f[sidx(N, h, 0, y, z) * C + c] += f[sidx(N, h, N, y, z) * C + c];
+    for (int z = 0; z <= N; ++z)
+        for (int x = 0; x < N; ++x)
+            for (int c = 0; c < C; ++c) f[sidx(N, h, x, 0, z) * C + c] += f[sidx(N, h, x, N, z) * C + c];
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x)
+            for (int c = 0; c < C; ++c) f[sidx(N, h, x, y, 0) * C + c] += f[sidx(N, h, x, y, N) * C + c];
+    return 0;
+}
+
+int orc_vector_ngp_comm(double *B, int N, int h) { return upper_comm(B, N, h, 3); }
+
+int orc_symtensor_ngp_comm(double *T, int N, int h) { return upper_comm(T, N, h, 6); }
+
 int orc_brute_deposit(const double *pos, const double *vel, const double *mass,
                       double mass_global, i64 np, int N, const double *stagger,
-                      int va, int vb, double *out) {
+                      int va, int vb, int ngp, double *out) {
     if (!pos || !out || !stagger || np < 0 || N < 1) return ORC_EBADARG;
     if ((va >= 0 || vb >= 0) && !vel) return ORC_EBADARG;
     const double res = 1.0 / N;
@@ -420,10 +570,22 @@ int orc_brute_deposit(const double *pos, const double *vel, const double *mass,
             n0[d] = (int)fl;
             f[d] = t - fl;
         }
+        /* nearest-grid-point axes: all weight on the nearest node of the staggered grid */
+        double wl[3][2];
+        for (int d = 0; d < 3; ++d) {
+            if ((ngp >> d) & 1) {
+                if (f[d] >= 0.5) n0[d] += 1;
+                wl[d][0] = 1.0;
+                wl[d][1] = 0.0;
+            } else {
+                wl[d][0] = 1.0 - f[d];
+                wl[d][1] = f[d];
+            }
+        }
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j)
                 for (int k = 0; k < 2; ++k) {
-                    const double w = (i ? f[0] : 1 - f[0]) * (j ? f[1] : 1 - f[1]) * (k ? f[2] : 1 - f[2]);
+                    const double w = wl[0][i] * wl[1][j] * wl[2][k];
                     const int x = ((n0[0] + i) % N + N) % N;
                     const int y = ((n0[1] + j) % N + N) % N;
                     const int z = ((n0[2] + k) % N + N) % N;

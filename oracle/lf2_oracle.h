@@ -59,6 +59,13 @@ int orc_vector_project(const double *pos, const double *vel, const double *mass,
                        double mass_global, orc_i64 np, double *B, int N, int h);
 int orc_vecvec_project(const double *pos, const double *vel, const double *mass,
                        double mass_global, orc_i64 np, double *T, int N, int h);
+/* the CIC-NGP hybrids:
+ *   vector_ngp    : vectorProjectionCICNGP_project    (particles/projections.hpp:422-549)
+ *   symtensor_ngp : symtensorProjectionCICNGP_project (particles/projections.hpp:709-847) */
+int orc_vector_ngp_project(const double *pos, const double *vel, const double *mass,
+                           double mass_global, orc_i64 np, double *B, int N, int h);
+int orc_symtensor_ngp_project(const double *pos, const double *vel, const double *mass,
+                              double mass_global, orc_i64 np, double *T, int N, int h);
 
 /* the periodic ghost folds that follow each deposit (the *_comm functions):
  *   scalar : particles/projections.hpp:194-300
@@ -67,16 +74,22 @@ int orc_vecvec_project(const double *pos, const double *vel, const double *mass,
 int orc_scalar_comm(double *rho, int N, int h);
 int orc_vector_comm(double *B, int N, int h);
 int orc_vecvec_comm(double *T, int N, int h);
+/*   vector_ngp    : particles/projections.hpp:558-681
+ *   symtensor_ngp : particles/projections.hpp:856-976 */
+int orc_vector_ngp_comm(double *B, int N, int h);
+int orc_symtensor_ngp_comm(double *T, int N, int h);
 
 /* Brute-force periodic CIC deposit straight from the hat-function definition:
  * value(node n) = sum_p q_p * prod_d hat(p_d*N - n_d - s_d), over all periodic
  * images, for a single component with stagger s (0 or 0.5 per axis).
  * q = (mass/res^3) * vfac where vfac = 1, v_a, or v_a*v_b (a,b = -1 for none).
+ * Axes in the bit mask `ngp` use the nearest-grid-point kernel instead of the hat:
+ * all of q on the node of the staggered grid nearest to the particle.
  * Writes the N^3 interior of a C=1 field without ghosts (out[x + N(y + N z)]).
  * Independent of the per-cell loop structure used by the *_project functions. */
 int orc_brute_deposit(const double *pos, const double *vel, const double *mass,
                       double mass_global, orc_i64 np, int N, const double *stagger,
-                      int va, int vb, double *out);
+                      int va, int vb, int ngp, double *out);
 
 /* 3-D R2C / C2R FFT with the PlanFFT layout (LATfield2_PlanFFT_decl.hpp:851-1056).
  * forward  : k = sum_x f(x) exp(-2 pi i k.x/N), unnormalised, input preserved;

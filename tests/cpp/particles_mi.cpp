@@ -2,7 +2,8 @@
 // every node with unit velocity (the initial condition of benchmarks_tests/
 // benchmarks_particles.cpp:131-149, with box length 1 and mass 0.1):
 //   - scalar, vector and VecVec CIC projections followed by their _comm are uniform
-//     (rho = m N^3, B_i = m v_i N^3, T_ij = m v_i v_j N^3) in the interior;
+//     (rho = m N^3, B_i = m v_i N^3, T_ij = m v_i v_j N^3) in the interior, and so are the
+//     CIC-NGP vector and symmetric tensor projections (projections.hpp:405-976);
 //   - the gathered force of a uniform potential is zero, so updateVel leaves v = 1 and
 //     returns |v| = sqrt(3);
 //   - moveParticles(gpu_drift, dt) moves every particle by dt in each direction;
@@ -84,6 +85,24 @@ int main(int argc, char **argv) {
     }
     std::printf("projections: max rel error rho %.2e B %.2e T %.2e\n", e_rho, e_b, e_t);
     if (e_rho > tol || e_b > tol || e_t > tol) ++fails;
+    // the CIC-NGP hybrids of the same lattice are uniform too
+    projection_init(&B);
+    projection_init(&T);
+    vectorProjectionCICNGP_project(&parts, &B);
+    vectorProjectionCICNGP_comm(&B);
+    symtensorProjectionCICNGP_project(&parts, &T);
+    symtensorProjectionCICNGP_comm(&T);
+    B.toHost();
+    T.toHost();
+    e_b = 0;
+    e_t = 0;
+    for (x.first(); x.test(); x.next()) {
+        for (int c = 0; c < 3; ++c) e_b = std::fmax(e_b, std::fabs(B(x, c) / want - 1));
+        for (int i = 0; i < 3; ++i)
+            for (int j = i; j < 3; ++j) e_t = std::fmax(e_t, std::fabs(T(x, i, j) / want - 1));
+    }
+    std::printf("CIC-NGP projections: max rel error B %.2e T %.2e\n", e_b, e_t);
+    if (e_b > tol || e_t > tol) ++fails;
     // uniform potential: no force
     for (x.first(); x.test(); x.next()) phi(x) = 3.0;
     phi.toDevice();

@@ -43,6 +43,21 @@ def deposit_fixture():
     np.savez(os.path.join(HERE, "deposit_n4.npz"), N=N, pos=pos, vel=vel, mass=mass, rho=rho, B=B, T=T)
 
 
+def deposit_ngp_fixture():
+    rng = np.random.default_rng(8)
+    N = 4
+    pos = rng.random((20, 3))
+    pos[0] = [0.0, 0.0, 0.0]          # on a node
+    pos[1] = [0.25, 0.5, 0.75]        # on a node
+    pos[2] = [0.125, 0.375, 0.9375]   # link / plaquette centres
+    pos[3] = [0.999, 0.001, 0.5]      # near the periodic faces
+    pos[4] = [0.2499, 0.7501, 0.0]    # either side of a cell face
+    vel = rng.standard_normal((20, 3))
+    mass = rng.random(20) + 0.5
+    B, T = numpy_pm.deposits_ngp(pos, vel, mass, N)
+    np.savez(os.path.join(HERE, "deposit_ngp_n4.npz"), N=N, pos=pos, vel=vel, mass=mass, B=B, T=T)
+
+
 def pm_fixture():
     rng = np.random.default_rng(11)
     N = 8
@@ -57,5 +72,6 @@ def pm_fixture():
 if __name__ == "__main__":
     fft_fixture()
     deposit_fixture()
+    deposit_ngp_fixture()
     pm_fixture()
     print("fixtures written to", HERE)

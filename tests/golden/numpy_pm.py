@@ -11,12 +11,18 @@ from __future__ import annotations
 import numpy as np
 
 
-def hat_deposit(pos, N, q, stagger=(0.0, 0.0, 0.0)):
-    """sum_p q_p prod_d hat(pos_d*N - n_d - s_d) over periodic images; [z,y,x]."""
+def hat_deposit(pos, N, q, stagger=(0.0, 0.0, 0.0), ngp=0):
+    """sum_p q_p prod_d hat(pos_d*N - n_d - s_d) over periodic images; [z,y,x].
+    Axes in the bit mask ``ngp`` take the nearest-grid-point kernel (the indicator
+    of |pos_d*N - n_d - s_d| <= 1/2, upper node on a tie) instead of the hat."""
     out = np.zeros((N, N, N))
     t = pos * N - np.asarray(stagger)[None, :]
     n0 = np.floor(t).astype(np.int64)
     f = t - n0
+    for d in range(3):
+        if (ngp >> d) & 1:
+            n0[:, d] = np.floor(t[:, d] + 0.5).astype(np.int64)
+            f[:, d] = 0.0
     for i in (0, 1):
         for j in (0, 1):
             for k in (0, 1):
@@ -98,3 +104,21 @@ def pm_step(pos, vel, mass, N, dt_kick, dt_drift):
     pos2 = np.where(pos2 < 0, pos2 + 1.0, pos2)
     pos2 = np.where(pos2 >= 1.0, pos2 - 1.0, pos2)
     return {"rho": rho, "phi": phi, "acc": acc, "vel": vel2, "pos": pos2}
+
+
+def deposits_ngp(pos, vel, mass, N):
+    """the CIC-NGP hybrids: B (3), component a on the link grid along a with the
+    nearest-grid-point kernel there and CIC elsewhere; T (6, symmetric order), T_aa
+    CIC on the vertices, T_ab (a<b) NGP on the links along a and b, CIC on the third."""
+    m = mass * float(N) ** 3
+    B = np.stack([hat_deposit(pos, N, m * vel[:, a], tuple(0.5 if d == a else 0.0 for d in range(3)),
+                              ngp=1 << a) for a in range(3)])
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    T = []
+    for a, b in pairs:
+        if a == b:
+            T.append(hat_deposit(pos, N, m * vel[:, a] * vel[:, a]))
+        else:
+            st = tuple(0.5 if d in (a, b) else 0.0 for d in range(3))
+            T.append(hat_deposit(pos, N, m * vel[:, a] * vel[:, b], st, ngp=(1 << a) | (1 << b)))
+    return B, np.stack(T)

@@ -122,6 +122,46 @@ def test_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mode):
         assert rel_l2(got, ref) <= tol, (mode, f.C, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-6)])
+@pytest.mark.parametrize("N,h,n,clustered,masses", [(16, 1, 5000, False, False), (32, 2, 40000, True, True),
+                                                     (64, 1, 6000, False, False),
+                                                     (16, 1, 120000, True, True),
+                                                     (20, 1, 8000, True, True),
+                                                     (14, 1, 500, False, False)])
+@pytest.mark.parametrize("mode", ["unfolded", "unfolded+comm", "periodic"])
+def test_cic_ngp_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mode):
+    """vectorProjectionCICNGP_* and symtensorProjectionCICNGP_* (projections.hpp:405-976)"""
+    announce(f"CIC-NGP deposits {prec} N={N} h={h} n={n} clustered={clustered} masses={masses} mode={mode}")
+    pos, vel, mass = _random_particles(n, 7 * N + n, clustered)
+    parts, _ = _upload(N, pos, vel, prec, mass if masses else None)
+    d = parts.download()
+    lat = lf.Lattice(3, N, h)
+    B, T = lf.Field(lat, 3, DT[prec]), lf.Field(lat, 6, DT[prec])
+    target = lf.DEP_PERIODIC if mode == "periodic" else lf.DEP_UNFOLDED
+    for f, kd, cm in ((B, lf.PROJ_VECTOR_NGP, lf.vectorProjectionCICNGP_comm),
+                      (T, lf.PROJ_SYMTENSOR_NGP, lf.symtensorProjectionCICNGP_comm)):
+        lf.projection_init(f)
+        parts.deposit(kd, f, target)
+        if mode == "unfolded+comm":
+            cm(f)
+    torch.cuda.synchronize()
+    m = d["mass"] if masses else None
+    rB, rT = orc.real_field(N, h, 3), orc.real_field(N, h, 6)
+    orc.vector_ngp_project(d["pos"], d["vel"], rB, N, h, mass=m)
+    orc.symtensor_ngp_project(d["pos"], d["vel"], rT, N, h, mass=m)
+    if mode != "unfolded":
+        orc.vector_ngp_comm(rB, N, h)
+        orc.symtensor_ngp_comm(rT, N, h)
+    G = N + 2 * h
+    for f, ref in ((B, rB), (T, rT)):
+        got = real_np(f)
+        if mode == "periodic":
+            got = got[h:G - h, h:G - h, h:G - h]
+            ref = ref[h:G - h, h:G - h, h:G - h]
+        for c in range(f.C):
+            assert rel_l2(got[..., c], ref[..., c]) <= tol, (mode, f.C, c, rel_l2(got[..., c], ref[..., c]))
+
+
 def test_fixed_point_deposit_is_reproducible_and_accurate():
     N = 32
     announce(f"fixed-point deposit N={N}")

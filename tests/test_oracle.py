@@ -296,6 +296,122 @@ def test_deposit_conservation_and_unfolded_ghosts():
     assert np.array_equal(rho_f[1:N + 1, 1:N + 1, N + 1], rho[1:N + 1, 1:N + 1, N + 1])
 
 
+# the CIC-NGP hybrids (particles/projections.hpp:405-976)
+_PAIRS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+
+def _deposit_ngp(pos, vel, mass, N, h=1, fold=True):
+    B = orc.real_field(N, h, 3)
+    T = orc.real_field(N, h, 6)
+    orc.vector_ngp_project(pos, vel, B, N, h, mass=mass)
+    orc.symtensor_ngp_project(pos, vel, T, N, h, mass=mass)
+    if fold:
+        orc.vector_ngp_comm(B, N, h)
+        orc.symtensor_ngp_comm(T, N, h)
+    return B, T
+
+
+def test_ngp_deposits_match_golden_fixture():
+    g = np.load(os.path.join(GOLD, "deposit_ngp_n4.npz"))
+    N = int(g["N"])
+    for h in (1, 2):
+        B, T = _deposit_ngp(g["pos"], g["vel"], g["mass"], N, h)
+        for a in range(3):
+            assert np.allclose(orc.interior(B, h)[..., a], g["B"][a], rtol=1e-12, atol=1e-9)
+        for c in range(6):
+            assert np.allclose(orc.interior(T, h)[..., c], g["T"][c], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("N", [8, 9])
+def test_ngp_folded_deposits_match_brute_force(N):
+    rng = np.random.default_rng(30 + N)
+    pos = rng.random((300, 3))
+    vel = rng.standard_normal((300, 3))
+    mass = rng.random(300) + 0.1
+    B, T = _deposit_ngp(pos, vel, mass, N)
+    for a in range(3):
+        st = tuple(0.5 if d == a else 0.0 for d in range(3))
+        ref = orc.brute_deposit(pos, N, st, vel=vel, va=a, mass=mass, ngp=1 << a)
+        assert np.allclose(orc.interior(B, 1)[..., a], ref, rtol=1e-12, atol=1e-9)
+    for c, (a, b) in enumerate(_PAIRS):
+        st = tuple(0.0 if a == b else (0.5 if d in (a, b) else 0.0) for d in range(3))
+        ngp = 0 if a == b else (1 << a) | (1 << b)
+        ref = orc.brute_deposit(pos, N, st, vel=vel, va=a, vb=b, mass=mass, ngp=ngp)
+        assert np.allclose(orc.interior(T, 1)[..., c], ref, rtol=1e-12, atol=1e-9)
+    # the diagonal of the hybrid tensor is the CIC VecVec diagonal; the hybrid vector
+    # differs from the CIC one (a plausible swap of the two kernels fails here)
+    _, Bc, Tc = _deposit_all(pos, vel, mass, N)
+    for c in (0, 3, 5):
+        assert np.allclose(T[..., c], Tc[..., c], rtol=1e-12, atol=1e-9)
+    assert not np.allclose(B, Bc)
+
+
+def test_ngp_single_particle_hand_computed():
+    N = 4
+    pos = np.array([[0.25 + 0.25 / 4, 0.5 + 0.5 / 4, 0.75 + 0.75 / 4]])  # cell (1,2,3), frac (1/4,1/2,3/4)
+    vel = np.array([[2.0, -3.0, 5.0]])
+    B, T = _deposit_ngp(pos, vel, None, N, fold=False)
+    f = (0.25, 0.5, 0.75)
+    c0 = (1, 2, 3)
+    q = N ** 3
+
+    def at(F, o, comp):
+        x, y, z = (c0[d] + o[d] for d in range(3))
+        return F[1 + z, 1 + y, 1 + x, comp]
+
+    def w1(d, up):
+        return f[d] if up else 1 - f[d]
+
+    for a in range(3):
+        o1, o2 = [d for d in range(3) if d != a]
+        for s1 in (0, 1):
+            for s2 in (0, 1):
+                o = [0, 0, 0]
+                o[o1], o[o2] = s1, s2
+                assert math.isclose(at(B, o, a), q * vel[0, a] * w1(o1, s1) * w1(o2, s2), rel_tol=1e-14)
+        assert math.isclose(B[..., a].sum(), q * vel[0, a], rel_tol=1e-14)
+    for c, (a, b) in enumerate(_PAIRS):
+        vv = q * vel[0, a] * vel[0, b]
+        if a == b:
+            for o in np.ndindex(2, 2, 2):
+                w = w1(0, o[0]) * w1(1, o[1]) * w1(2, o[2])
+                assert math.isclose(at(T, o, c), vv * w, rel_tol=1e-14)
+        else:
+            t = 3 - a - b
+            for s in (0, 1):
+                o = [0, 0, 0]
+                o[t] = s
+                assert math.isclose(at(T, o, c), vv * w1(t, s), rel_tol=1e-14)
+        assert math.isclose(T[..., c].sum(), vv, rel_tol=1e-14)
+
+
+def test_ngp_conservation_and_upper_ghosts_only():
+    N = 8
+    rng = np.random.default_rng(3)
+    pos = rng.random((500, 3))
+    vel = rng.standard_normal((500, 3))
+    B, T = _deposit_ngp(pos, vel, None, N, fold=False)
+    vol = 1.0 / N ** 3
+    # the hybrids never write a lower ghost
+    for F in (B, T):
+        assert np.all(F[0] == 0) and np.all(F[:, 0] == 0) and np.all(F[:, :, 0] == 0)
+    for a in range(3):
+        assert math.isclose(B[..., a].sum() * vol, vel[:, a].sum(), rel_tol=1e-10, abs_tol=1e-10)
+    for c, (a, b) in enumerate(_PAIRS):
+        assert math.isclose(T[..., c].sum() * vol, (vel[:, a] * vel[:, b]).sum(), rel_tol=1e-10, abs_tol=1e-10)
+    # the folds are the scalar fold on each component
+    for F, fold, C in ((B, orc.vector_ngp_comm, 3), (T, orc.symtensor_ngp_comm, 6)):
+        G = F.copy()
+        fold(G, N, 1)
+        for c in range(C):
+            r = F[..., c:c + 1].copy()
+            orc.scalar_comm(r, N, 1)
+            assert np.array_equal(G[..., c], r[..., 0])
+        for c, (a, b) in enumerate(_PAIRS[:C] if C == 6 else [(0, 0), (1, 1), (2, 2)]):
+            tot = (vel[:, a] * vel[:, b]).sum() if C == 6 else vel[:, c].sum()
+            assert math.isclose(orc.interior(G, 1)[..., c].sum() * vol, tot, rel_tol=1e-10, abs_tol=1e-10)
+
+
 def test_particles_on_nodes_give_uniform_density():
     """benchmarks_tests/benchmarks_particles.cpp:131-149 places one particle per node"""
     N = 8
