@@ -461,6 +461,13 @@ class Particles {
         }
     }
 
+    // the vector a vector or tensor projection uses (projections.hpp:453-459): without oset
+    // the velocity, with oset the Real[3] at byte offset oset[1] of every particle
+    void select_vector(size_t *oset) {
+        sync();
+        lf2mi_or_die(lf2mi_parts_vector_from(h_, oset ? (int64_t)oset[1] : -1, lf2mi_real_code()), "projection vector");
+    }
+
   private:
     // the staged structs go to the device whole (a payload of sizeof(part) bytes moves with
     // each particle), read by their members' offsets
@@ -526,6 +533,7 @@ template <typename part, typename part_info, typename part_dataType>
 void vectorProjectionCIC_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *B, size_t *oset = NULL,
                                  int flag_where = FROM_INFO) {
     parts->select_mass(oset, flag_where);
+    parts->select_vector(oset);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECTOR, B->device_ptr(), B->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "vectorProjectionCIC_project");
@@ -539,6 +547,7 @@ template <typename part, typename part_info, typename part_dataType>
 void VecVecProjectionCIC_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *T, size_t *oset = NULL,
                                  int flag_where = FROM_INFO) {
     parts->select_mass(oset, flag_where);
+    parts->select_vector(oset);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECVEC, T->device_ptr(), T->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "VecVecProjectionCIC_project");
@@ -553,6 +562,7 @@ template <typename part, typename part_info, typename part_dataType>
 void vectorProjectionCICNGP_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *vel,
                                     size_t *oset = NULL, int flag_where = FROM_INFO) {
     parts->select_mass(oset, flag_where);
+    parts->select_vector(oset);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_VECTOR_NGP, vel->device_ptr(), vel->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "vectorProjectionCICNGP_project");
@@ -566,6 +576,7 @@ template <typename part, typename part_info, typename part_dataType>
 void symtensorProjectionCICNGP_project(Particles<part, part_info, part_dataType> *parts, Field<Real> *Tij,
                                        size_t *oset = NULL, int flag_where = FROM_INFO) {
     parts->select_mass(oset, flag_where);
+    parts->select_vector(oset);
     lf2mi_or_die(lf2mi_deposit(parts->handle(), LF2MI_PROJ_SYMTENSOR_NGP, Tij->device_ptr(), Tij->lattice().halo(),
                                LF2MI_DEP_UNFOLDED, LF2MI_ACC_NATIVE, nullptr),
                  "symtensorProjectionCICNGP_project");

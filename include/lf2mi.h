@@ -232,6 +232,15 @@ int lf2mi_parts_download_aos(lf2mi_parts p, void *base, int64_t n_room, int64_t 
 #define LF2MI_MASS_PART 1
 int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream);
 
+/* The vector a projection's oset[1] names (the vector and tensor projections read v at that
+ * byte offset of each part: projections.hpp:453-459, :741-747, :1019-1025, :1394-1400):
+ * offset -1, or the vel offset given to lf2mi_parts_upload_aos, selects the
+ * container's velocity (the default); any other offset selects the three Reals of type
+ * real_dtype at that offset of the payload, gathered on the deposit's stream by every later
+ * deposit of a vector or tensor kind.  The payload moves with its particles, so the choice
+ * holds across moves. */
+int lf2mi_parts_vector_from(lf2mi_parts p, int64_t offset, int real_dtype);
+
 /* per-cell particle counts into a device int32 array of N^3 entries (bit-exact with
  * the histogram of Particles::getPartCoord keys) */
 int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream);

@@ -314,6 +314,12 @@ class Particles:
         stand)"""
         call("lf2mi_parts_mass_from", self.h, int(where), int(offset), _stream())
 
+    def vector_from(self, offset: int = -1, real: int = F32):
+        """vector of later vector/tensor deposits (a projection's oset[1]): the velocity
+        (offset -1 or the uploaded vel offset), else the 3 Reals of type `real` at byte
+        offset of each kept struct"""
+        call("lf2mi_parts_vector_from", self.h, int(offset), int(real))
+
     def cell_counts(self):
         torch = _torch()
         c = torch.empty(self.N ** 3, dtype=torch.int32, device="cuda")

@@ -1783,6 +1783,10 @@ struct lf2mi_parts_s {
     int64_t bytes;
     int plw;       // payload words (8 bytes) per particle, 0 without a payload
     int mass_src;  // deposits use: 0 the global mass, 1 the per-particle masses
+    int64_t vec_off;  // vector deposits use: -1 the velocity, else the Real[3] at this payload offset
+    int vec_real;     // that vector's Real type
+    int64_t aos_vel;  // offset of vel in the uploaded structs (-1: none uploaded)
+    void *vsrc;       // 3 planes of cap values (container precision) for the gathered vector
 };
 
 namespace lf2mi {
@@ -1922,6 +1926,38 @@ __global__ void payload_mass_kernel(int64_t np, const unsigned long long *__rest
     if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
 }
 
+// the Real[3] at byte `off` of each payload into three planes of `cap` values
+template <typename T, typename R>
+__global__ void payload_vec_kernel(int64_t np, const unsigned long long *__restrict__ pl, int plw, int64_t off,
+                                   T *__restrict__ out, int64_t cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(pl + i * plw) + off;
+    R v[3];
+    memcpy(v, b, sizeof v);
+    out[i] = (T)v[0];
+    out[cap + i] = (T)v[1];
+    out[2 * cap + i] = (T)v[2];
+}
+
+// point the deposit's velocity planes at the vector selected by lf2mi_parts_vector_from
+// (gathered from the payload on this stream)
+template <typename T>
+static int vector_source(lf2mi_parts_s *p, const unsigned long long *pl, T *(&vel)[3], hipStream_t st) {
+    if (p->vec_off < 0) return LF2MI_OK;
+    const int64_t cap = std::max<int64_t>(p->cap, 1);
+    if (!p->vsrc)
+        if (int rc = dalloc(p, &p->vsrc, (size_t)cap * 3 * sizeof(T))) return rc;
+    T *out = static_cast<T *>(p->vsrc);
+    if (p->vec_real == LF2MI_F32)
+        payload_vec_kernel<T, float><<<grid1(p->np), 256, 0, st>>>(p->np, pl, p->plw, p->vec_off, out, cap);
+    else
+        payload_vec_kernel<T, double><<<grid1(p->np), 256, 0, st>>>(p->np, pl, p->plw, p->vec_off, out, cap);
+    LF2MI_LAUNCHED();
+    for (int d = 0; d < 3; ++d) vel[d] = out + d * cap;
+    return LF2MI_OK;
+}
+
 }  // namespace lf2mi
 
 extern "C" {
@@ -1944,6 +1980,10 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
     p->cur = 0;
     p->plw = 0;
     p->mass_src = (flags & LF2MI_PARTS_MASS) ? 1 : 0;
+    p->vec_off = -1;
+    p->vec_real = LF2MI_F32;
+    p->aos_vel = -1;
+    p->vsrc = nullptr;
     int B = 1;
     for (int b = MAXB; b >= 1; --b)
         if (N % b == 0) { B = b; break; }
@@ -2240,6 +2280,7 @@ int lf2mi_parts_upload_aos(lf2mi_parts p, int64_t n, const void *base, int64_t s
         if (off_id >= 0) memcpy(&id[(size_t)i], q + off_id, 8);
         if (off_mass >= 0) memcpy(&mass[(size_t)i], q + off_mass, 8);
     }
+    if (p->plw) p->aos_vel = off_vel;
     return upload_impl(p, n, pos.data(), vel.data(), off_mass >= 0 ? mass.data() : nullptr,
                        off_id >= 0 ? id.data() : nullptr, p->plw ? b : nullptr, kept, stream);
 }
@@ -2339,6 +2380,22 @@ int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream
     if (rc) return rc;
     p->mass_src = 1;
     p->total_mass = total;
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_vector_from(lf2mi_parts p, int64_t offset, int real_dtype) {
+    if (!p) return fail(LF2MI_EINVAL, "null container");
+    // the payload's copy of vel is the uploaded one: the kicked velocity lives in the container
+    if (offset < 0 || offset == p->aos_vel) {
+        p->vec_off = -1;
+        return LF2MI_OK;
+    }
+    if (real_dtype != LF2MI_F32 && real_dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad real dtype %d", real_dtype);
+    const int64_t rs = real_dtype == LF2MI_F32 ? 4 : 8;
+    if (!p->plw || offset + 3 * rs > (int64_t)p->plw * 8)
+        return fail(LF2MI_EINVAL, "vector offset %lld outside the payload", (long long)offset);
+    p->vec_off = offset;
+    p->vec_real = real_dtype;
     return LF2MI_OK;
 }
 
@@ -2508,6 +2565,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         A.tgt = target;
         if (p->mass_src && p->b64.mass) A.mass_nv = nv;
         else A.s.mass = nullptr;
+        if (kind != LF2MI_PROJ_SCALAR)
+            if (int rc = vector_source(p, p->b64.pl, A.s.vel, st)) return rc;
         const unsigned g = grid1(p->np);
         static const char *const names64[5] = {"deposit_f64_scalar", "deposit_f64_vector", "deposit_f64_vecvec",
                                                "deposit_f64_vector_ngp", "deposit_f64_symtensor_ngp"};
@@ -2532,6 +2591,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     const bool pm = p->mass_src && A.s.mass;
     if (!pm) A.s.mass = nullptr;
     A.mass_nv = (float)(pm ? nv : p->mass_global * nv);
+    if (kind != LF2MI_PROJ_SCALAR)
+        if (int rc = vector_source(p, A.s.pl, A.s.vel, st)) return rc;
     A.field = (float *)field;
     A.h = halo;
     A.tgt = target;

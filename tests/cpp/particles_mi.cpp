@@ -27,6 +27,7 @@ struct part_weighted {
     Real vel[3];
     double mass;
     double weight;
+    Real mom[3];
 };
 
 // sum of the interior of a scalar field (host copy)
@@ -130,7 +131,7 @@ int main(int argc, char **argv) {
     {
         Particles<part_weighted, part_simple_info, part_simple_dataType> wp;
         wp.initialize(info, dtype, &lat_part, box);
-        double msum = 0.0, wsum = 0.0;
+        double msum = 0.0, wsum = 0.0, momsum[3] = {0.0, 0.0, 0.0};
         long k = 0;
         for (xp.first(); xp.test(); xp.next()) {
             part_weighted p;
@@ -141,6 +142,10 @@ int main(int argc, char **argv) {
             }
             p.mass = 0.2 + 0.01 * (k % 5);
             p.weight = 0.3 + 0.02 * (k % 3);
+            for (int d = 0; d < 3; ++d) {
+                p.mom[d] = (Real)(1.0 + 0.125 * (k % 7) + d);
+                momsum[d] += p.mom[d];
+            }
             msum += p.mass;
             wsum += p.weight;
             wp.addParticle_global(p);
@@ -166,6 +171,28 @@ int main(int argc, char **argv) {
         if (std::fabs(s_mass / msum - 1) > stol || std::fabs(s_weight / wsum - 1) > stol ||
             std::fabs(s_info / (0.1 * k) - 1) > stol)
             ++fails;
+        // vector projections: the velocity with the individual mass by default; any Real[3]
+        // member with oset[1] (projections.hpp:453-459, :1019-1025)
+        auto component_sum = [&](int c) {
+            B.toHost();
+            Site y(lat);
+            double t = 0.0;
+            for (y.first(); y.test(); y.next()) t += B(y, c);
+            return t / nv;
+        };
+        projection_init(&B);
+        vectorProjectionCIC_project(&wp, &B);
+        vectorProjectionCIC_comm(&B);
+        const double b_vel = component_sum(1);
+        size_t os[2] = {om, offsetof(part_weighted, mom)};
+        projection_init(&B);
+        vectorProjectionCICNGP_project(&wp, &B, os, FROM_INFO);
+        vectorProjectionCICNGP_comm(&B);
+        double b_err = std::fabs(b_vel / (0.5 * msum) - 1);
+        for (int c = 0; c < 3; ++c) b_err = std::fmax(b_err, std::fabs(component_sum(c) / (0.1 * momsum[c]) - 1));
+        std::printf("struct vectors: default (vel, individual mass) and oset = {info mass, mom}: max rel error %.2e\n",
+                    b_err);
+        if (b_err > stol) ++fails;
         wp.moveParticles(gpu_drift(), 3.0 / N);  // 1.5 cells: many cross a brick face
         std::vector<part_weighted> back;
         wp.getParticles(back);

@@ -1,7 +1,8 @@
 """GPU parity of user particle structs (Particles<part, ...> with any part type,
 particles/LATfield2_Particles.hpp:131-344): the container keeps each particle's struct and
 moves it through the re-bucketing; the deposits take the mass from part_info or from a
-double at any byte offset of the struct (projections.hpp:108-126, :144-152).
+double at any byte offset of the struct (projections.hpp:108-126, :144-152), and the vector
+and tensor deposits their vector from any Real[3] of it (oset[1]).
 
 Bars: kept struct members bit-exact by ID after moves; positions and velocities as in
 lf2mi_parts_download; deposits rel-L2 <= 1e-6 (FP32) / 1e-12 (FP64) against the FP64 oracle
@@ -130,6 +131,68 @@ def test_structs_follow_their_particles_through_moves(prec):
         assert rel_l2(got, _oracle(np.ascontiguousarray(d["pos"][order]), ref["w"], N)) <= TOL[prec], step
 
 
+_VEC_KINDS = [(lf.PROJ_VECTOR, 3, "vector_project"), (lf.PROJ_VECVEC, 6, "vecvec_project"),
+              (lf.PROJ_VECTOR_NGP, 3, "vector_ngp_project"), (lf.PROJ_SYMTENSOR_NGP, 6, "symtensor_ngp_project")]
+
+
+def _vector_deposits(parts, N, prec, pos, vec, tol, what):
+    lat = lf.Lattice(3, N, 1)
+    s = slice(1, N + 1)
+    for kind, C, fn in _VEC_KINDS:
+        f = lf.Field(lat, C, DT[prec])
+        f.zero()
+        parts.deposit(kind, f, lf.DEP_PERIODIC)
+        torch.cuda.synchronize()
+        ref = orc.real_field(N, 1, C)
+        getattr(orc, fn)(np.ascontiguousarray(pos), np.ascontiguousarray(vec), ref, N, 1, mass_global=0.7)
+        getattr(orc, fn.replace("project", "comm"))(ref, N, 1)
+        got = real_np(f)[s, s, s]
+        for c in range(C):
+            assert rel_l2(got[..., c], ref[s, s, s, c]) <= tol, (what, fn, c)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("real", ["<f8", "<f4"])
+def test_vector_projections_of_any_struct_member(prec, real):
+    """a projection's oset[1] (projections.hpp:453-459, :741-747, :1019-1025, :1394-1400)"""
+    N, n = 20, 5000
+    announce(f"structs vector member {prec} Real={real} N={N} n={n}")
+    dt = np.dtype([("ID", "<i8"), ("pos", real, 3), ("vel", real, 3), ("mom", real, 3), ("mass", "<f8")], align=True)
+    rng = np.random.default_rng(17)
+    a = np.zeros(n, dtype=dt)
+    a["ID"] = np.arange(n)
+    a["pos"] = rng.random((n, 3))
+    a["vel"] = rng.standard_normal((n, 3))
+    a["mom"] = rng.standard_normal((n, 3)) * 3.0 + 1.0
+    parts = lf.Particles(N, n, DT[prec], ids=True, mass=0.7)
+    parts.set_payload(dt.itemsize)
+    assert parts.upload_structs(a) == n
+    parts.mass_from(lf.MASS_INFO)
+    code = lf.F32 if real == "<f4" else lf.F64
+    # a drift re-buckets the particles (and their structs) first
+    parts.gather_kick_drift(None, dt_kick=0.0, dt_drift=0.3 / N)
+    parts.vector_from(dt.fields["mom"][1], code)
+    d = parts.download()
+    order = np.argsort(d["id"])
+    pos = d["pos"][order]
+    mom = _by_id(parts.download_structs(dt))["mom"].astype(np.float64)
+    _vector_deposits(parts, N, prec, pos, mom, TOL[prec], "mom")
+    # a kick changes the container's velocity; the vel offset selects that velocity, not the
+    # uploaded copy in the struct
+    lat = lf.Lattice(3, N, 1)
+    phi = lf.Field(lat, 1, DT[prec])
+    g = torch.Generator().manual_seed(3)
+    phi.t.copy_(torch.randn(phi.t.shape, generator=g, dtype=torch.float64).to(phi.t.dtype))
+    parts.gather_kick_drift(phi, dt_kick=0.05, dt_drift=0.0)
+    parts.vector_from(dt.fields["vel"][1], code)
+    d = parts.download()
+    order = np.argsort(d["id"])
+    assert not np.allclose(d["vel"][order], a["vel"].astype(np.float64))
+    _vector_deposits(parts, N, prec, d["pos"][order], d["vel"][order], TOL[prec], "vel")
+    parts.vector_from(-1)
+    _vector_deposits(parts, N, prec, d["pos"][order], d["vel"][order], TOL[prec], "default")
+
+
 def test_struct_errors():
     announce("structs errors")
     a = _structs(100, 1, "<f8")
@@ -145,6 +208,12 @@ def test_struct_errors():
     parts.upload_structs(a)
     with pytest.raises(lf.Lf2miError):
         parts.set_payload(a.dtype.itemsize)  # only before the first upload
+    with pytest.raises(lf.Lf2miError):
+        parts.vector_from(a.dtype.itemsize - 8, lf.F64)  # 24 bytes past the struct
+    with pytest.raises(lf.Lf2miError):
+        parts.vector_from(0, 7)  # bad Real type
     plain = lf.Particles(16, 100, lf.F32, ids=True)
     with pytest.raises(lf.Lf2miError):
         plain.mass_from(lf.MASS_PART, 0)  # no masses and no payload
+    with pytest.raises(lf.Lf2miError):
+        plain.vector_from(0, lf.F32)  # no payload
