@@ -4,11 +4,14 @@
  *
  * Same names and argument meaning as the reference (namespace LATfield2):
  *   Lattice                      LATfield2_Lattice_decl.hpp:13-218
- *   Site, rKSite                 LATfield2_Site_decl.hpp:13-277 (rKSite dims {0,2,1})
+ *   Site, rKSite, cKSite         LATfield2_Site_decl.hpp:13-277 (rKSite dims {0,2,1},
+ *                                cKSite dims {1,2,0})
  *   Imag                         Imag.hpp:24-110
  *   Field<Real>, Field<Imag>     LATfield2_Field_decl.hpp:61-430 (AoS, symmetric 3x3 -> 6)
  *   Field::updateHalo            LATfield2_Field_decl.hpp:889-935
- *   PlanFFT<Imag>, execute       LATfield2_PlanFFT_decl.hpp:83-1203 (R2C)
+ *   PlanFFT<Imag>, execute       LATfield2_PlanFFT_decl.hpp:83-1203 (R2C and C2C; the
+ *                                C2C backward is the inverse of the forward, which the
+ *                                reference's single-rank backward is not)
  *   Particles<part,info,dtype>   particles/LATfield2_Particles.hpp:131-344
  *   projection_init, scalar/vector/VecVec ProjectionCIC _project/_comm
  *                                particles/projections.hpp:31-1814
@@ -124,6 +127,14 @@ class Lattice {
         int s[3] = {lat_real.size(0) / 2 + 1, lat_real.size(0), lat_real.size(0)};
         initialize(3, s, halo);
     }
+    // N^3 k-lattice of the C2C transform (LATfield2_Lattice.hpp:260-297)
+    void initializeComplexFFT(Lattice &lat_real, int halo) {
+        if (lat_real.dim() != 3 || lat_real.size(0) != lat_real.size(1) || lat_real.size(1) != lat_real.size(2)) {
+            std::fprintf(stderr, "Latfield2d::Lattice::initializeRealFFT : fft curently work only for 3d cubic lattice\n");
+            std::exit(-1);
+        }
+        initialize(3, lat_real.size(0), halo);
+    }
 
     int dim() const { return dim_; }
     int size(int i) const { return size_[i]; }
@@ -221,6 +232,26 @@ class rKSite : public Site {
     }
 };
 
+// C2C k-lattice cursor: asked direction (0,1,2) = (kx, ky, kz) maps to storage dims (1,2,0)
+// (LATfield2_Site.hpp:219-263)
+class cKSite : public Site {
+  public:
+    cKSite() : Site() {}
+    explicit cKSite(Lattice &lat) : Site(lat) {}
+    int coord(int dir) const { return c_[map(dir)]; }
+    int coordLocal(int dir) const { return c_[map(dir)]; }
+    cKSite operator+(int dir) const { return cks(Site::operator+(map(dir))); }
+    cKSite operator-(int dir) const { return cks(Site::operator-(map(dir))); }
+
+  private:
+    static int map(int dir) { return dir == 0 ? 1 : (dir == 1 ? 2 : 0); }
+    static cKSite cks(const Site &s) {
+        cKSite r;
+        static_cast<Site &>(r) = s;
+        return r;
+    }
+};
+
 // ------------------------------------------------------------------ Field
 template <class FieldType>
 class Field {
@@ -311,8 +342,12 @@ class Field {
 template <class compType>
 class PlanFFT {
   public:
-    PlanFFT() : plan_(nullptr), r_(nullptr), k_(nullptr), stage_(nullptr), mem_(FFT_OUT_OF_PLACE) {}
+    PlanFFT() : plan_(nullptr), r_(nullptr), c_(nullptr), k_(nullptr), stage_(nullptr), mem_(FFT_OUT_OF_PLACE) {}
     PlanFFT(Field<Real> *rfield, Field<compType> *kfield, const int mem_type = FFT_OUT_OF_PLACE) : PlanFFT() {
+        initialize(rfield, kfield, mem_type);
+    }
+    // C2C of two complex fields (LATfield2_PlanFFT_decl.hpp:600-719)
+    PlanFFT(Field<compType> *rfield, Field<compType> *kfield, const int mem_type = FFT_OUT_OF_PLACE) : PlanFFT() {
         initialize(rfield, kfield, mem_type);
     }
     ~PlanFFT() {
@@ -337,15 +372,32 @@ class PlanFFT {
                                            kfield->lattice().halo(), lf2mi_real_code(), &plan_),
                      "PlanFFT::initialize");
     }
+    void initialize(Field<compType> *rfield, Field<compType> *kfield, const int mem_type = FFT_OUT_OF_PLACE) {
+        if (rfield->components() != kfield->components()) {
+            std::fprintf(stderr, "Latfield2d::PlanFFT::initialize : fft curently work only for fields with same number of components\n");
+            std::exit(-1);
+        }
+        c_ = rfield;
+        k_ = kfield;
+        mem_ = mem_type;
+        rfield->alloc();
+        kfield->alloc();
+        lf2mi_or_die(lf2mi_fft_plan_create_c2c(rfield->lattice().size(0), rfield->components(), rfield->lattice().halo(),
+                                               kfield->lattice().halo(), lf2mi_real_code(), &plan_),
+                     "PlanFFT::initialize");
+    }
     // FFT_FORWARD: unnormalised e^{-2 pi i k.x/N}; FFT_BACKWARD: unnormalised e^{+}, divide
-    // by N^3 yourself (examples_fft.dox:123); other values do nothing, as in the reference
+    // by N^3 yourself (examples_fft.dox:123); other values do nothing, as in the reference.
+    // C2C: BACKWARD is the inverse of FORWARD (times N^3) on every input.
     void execute(int fft_type) {
-        lf2mi_or_die(lf2mi_fft_execute(plan_, fft_type, r_->device_ptr(), k_->device_ptr(), nullptr), "PlanFFT::execute");
+        void *r = r_ ? r_->device_ptr() : c_->device_ptr();
+        lf2mi_or_die(lf2mi_fft_execute(plan_, fft_type, r, k_->device_ptr(), nullptr), "PlanFFT::execute");
     }
 
   private:
     lf2mi_fft_plan plan_;
     Field<Real> *r_;
+    Field<compType> *c_;
     Field<compType> *k_;
     void *stage_;
     int mem_;

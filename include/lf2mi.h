@@ -128,6 +128,11 @@ typedef struct lf2mi_fft_plan_s *lf2mi_fft_plan;
  * components, real ghost width rhalo, k ghost width khalo.  Owns its twiddle tables
  * and workspace on the current device. */
 int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan);
+/* PlanFFT<Imag>(Field<Imag>*, Field<Imag>*) for the C2C case
+ * (LATfield2_PlanFFT_decl.hpp:608-719): same sizes; the k-field lattice is N^3
+ * (Lattice::initializeComplexFFT, LATfield2_Lattice.hpp:260-297) and holds mode
+ * (kx, ky, kz) at storage (kz, kx, ky) as cKSite reads it (LATfield2_Site.hpp:219-220). */
+int lf2mi_fft_plan_create_c2c(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan);
 int lf2mi_fft_plan_destroy(lf2mi_fft_plan plan);
 /* bytes of device memory the plan holds */
 int lf2mi_fft_plan_bytes(lf2mi_fft_plan plan, int64_t *bytes);
@@ -136,7 +141,12 @@ int lf2mi_fft_plan_bytes(lf2mi_fft_plan plan, int64_t *bytes);
  * e^{-2 pi i k.x/N} from the real interior into the k-field interior (input kept).
  * BACKWARD: unnormalised e^{+}, k-field kept, real interior written (ghosts untouched);
  * only the real parts of the kx = 0 and kx = N/2 modes are used, as in FFTW's c2r.
- * Any other direction is a no-op, as in the reference. */
+ * Any other direction is a no-op, as in the reference.
+ * C2C plans (:1059-1198): FORWARD is the unnormalised e^{-} DFT of the complex interior
+ * into the k-field interior (input kept); BACKWARD is the unnormalised e^{+} DFT back
+ * (k-field kept), so BACKWARD(FORWARD(f)) = N^3 f.  The reference's single-rank backward
+ * differs: it skips the transposition before its second pass (:1165-1167), so it applies
+ * the z transform twice and no y transform. */
 int lf2mi_fft_execute(lf2mi_fft_plan plan, int direction, void *rfield, void *kfield, void *stream);
 
 /* Green's-function multiply of examples/poissonSolver.cpp:99-111 on a k-field:

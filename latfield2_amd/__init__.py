@@ -25,7 +25,7 @@ from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64
                    PROJ_VECTOR, PROJ_VECTOR_NGP, PROJ_VECVEC, Lf2miError, call, load)
 
 __all__ = [
-    "Lattice", "Field", "KField", "PlanFFT", "Particles", "projection_init",
+    "Lattice", "Field", "KField", "CField", "PlanFFT", "Particles", "projection_init",
     "scalarProjectionCIC_project", "scalarProjectionCIC_comm", "vectorProjectionCIC_project",
     "vectorProjectionCIC_comm", "VecVecProjectionCIC_project", "VecVecProjectionCIC_comm",
     "vectorProjectionCICNGP_project", "vectorProjectionCICNGP_comm", "symtensorProjectionCICNGP_project",
@@ -155,17 +155,57 @@ class KField:
         return _torch().view_as_complex(self.t)
 
 
-class PlanFFT:
-    """PlanFFT<Imag>(Field<Real>*, Field<Imag>*) -- R2C/C2R on the device."""
+class CField:
+    """Field<Imag> on an N^3 lattice with C components: storage [d2, d1, d0, c, re/im] with
+    ghosts.  Position space is [z, y, x]; the C2C k-lattice (Lattice::initializeComplexFFT)
+    is [ky, kx, kz] (cKSite, LATfield2_Site.hpp:219-263)."""
 
-    def __init__(self, rfield: Field, kfield: KField | None = None):
+    def __init__(self, lat: Lattice, components: int = 1, dtype=F32, device: str = "cuda"):
+        torch = _torch()
+        self.lat, self.C, self.code = lat, int(components), _dtype_code(dtype)
+        G = lat.N + 2 * lat.halo
+        self.t = torch.zeros((G, G, G, self.C, 2), dtype=_torch_dtype(self.code), device=device)
+
+    @property
+    def N(self):
+        return self.lat.N
+
+    @property
+    def halo(self):
+        return self.lat.halo
+
+    @property
+    def kh(self):
+        return self.lat.halo
+
+    def data(self):
+        return self.t
+
+    def interior(self):
+        h, G = self.halo, self.N + 2 * self.halo
+        return self.t[h:G - h, h:G - h, h:G - h]
+
+    def as_complex(self):
+        return _torch().view_as_complex(self.t)
+
+
+class PlanFFT:
+    """PlanFFT<Imag>(Field<Real>*, Field<Imag>*) -- R2C/C2R on the device; with two
+    complex fields (CField, CField) the C2C transform (LATfield2_PlanFFT_decl.hpp:600-719),
+    whose backward is the inverse of its forward (times N^3)."""
+
+    def __init__(self, rfield, kfield=None):
         if kfield is not None and (kfield.N != rfield.N or kfield.C != rfield.C or kfield.code != rfield.code):
             raise ValueError("PlanFFT: real and k fields disagree on size, components or precision")
         self.r, self.k = rfield, kfield
         self.N, self.C, self.code = rfield.N, rfield.C, rfield.code
         kh = kfield.kh if kfield is not None else 0
+        self.c2c = isinstance(rfield, CField)
+        if self.c2c and not isinstance(kfield, CField):
+            raise ValueError("PlanFFT: a complex position-space field needs a C2C k-field (CField)")
         h = ctypes.c_void_p()
-        call("lf2mi_fft_plan_create", self.N, self.C, rfield.halo, kh, self.code, ctypes.byref(h))
+        call("lf2mi_fft_plan_create_c2c" if self.c2c else "lf2mi_fft_plan_create", self.N, self.C, rfield.halo, kh,
+             self.code, ctypes.byref(h))
         self.h = h
 
     def execute(self, fft_type: int):

@@ -48,6 +48,7 @@ SIGNATURES = {
     "lf2mi_field_sum": [_vp, _i, _i, _i, _i, _i, _dp, _vp],
     "lf2mi_field_affine": [_vp, _i, _i, _i, _i, _d, _d, _i, _vp],
     "lf2mi_fft_plan_create": [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
+    "lf2mi_fft_plan_create_c2c": [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
     "lf2mi_fft_plan_destroy": [_vp],
     "lf2mi_fft_plan_bytes": [_vp, _i64p],
     "lf2mi_fft_execute": [_vp, _i, _vp, _vp, _vp],

@@ -1,5 +1,6 @@
 // 3-D R2C / C2R FFT of the lf2mi library (PlanFFT<Imag> for the R2C case,
-// LATfield2_PlanFFT_decl.hpp:731-1056), written for gfx950.
+// LATfield2_PlanFFT_decl.hpp:731-1056), and the C2C transform of two complex fields
+// (:608-719, :1059-1198; passes described at fft_x_c2c_kernel), written for gfx950.
 //
 // Layout contract (same as the reference k-field): k[((kx+kh) + R((kz+kh) + KG(ky+kh)))C + c]
 // with R = N/2+1+2kh, KG = N+2kh.  Rows of the k-field are indexed by two "slots":
@@ -138,9 +139,11 @@ __device__ __forceinline__ void pass_g(const cplx<T> *__restrict__ tw, Load &&lo
 }
 
 // Whole transform of a tile.  FIRST_LDS / LAST_LDS say whether load0 reads and storeLast
-// writes the LDS tile (lds[e*ES + l]); otherwise they are HBM accessors.
-template <typename T, int N, int L, int ES, int WG, int S, int MAP, bool FIRST_LDS, bool LAST_LDS, class Load0,
-          class StoreLast>
+// writes the LDS tile (lds[e*ES + l]); otherwise they are HBM accessors.  MAPF / MAPL map the
+// threads of the first / last pass (default MAP): a transposing pass reads line-fastest and
+// writes element-fastest, or the other way round, straight from and to HBM.
+template <typename T, int N, int L, int ES, int WG, int S, int MAP, bool FIRST_LDS, bool LAST_LDS, int MAPF = MAP,
+          int MAPL = MAP, class Load0, class StoreLast>
 __device__ __forceinline__ void tile_fft_io(cplx<T> *lds, const cplx<T> *__restrict__ tw, Load0 &&load0,
                                             StoreLast &&storeLast) {
     using V = cplx<T>;
@@ -148,9 +151,9 @@ __device__ __forceinline__ void tile_fft_io(cplx<T> *lds, const cplx<T> *__restr
     auto st = [lds](int l, int e, V v) { lds[e * ES + l] = v; };
     constexpr int NP = fft_np(N);
     if constexpr (NP == 1) {
-        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAP, FIRST_LDS && LAST_LDS>(tw, load0, storeLast);
+        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAPF, FIRST_LDS && LAST_LDS>(tw, load0, storeLast);
     } else {
-        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAP, FIRST_LDS>(tw, load0, st);
+        pass_g<T, N, L, WG, S, fft_radix(N, 0), 1, MAPF, FIRST_LDS>(tw, load0, st);
         __syncthreads();
         if constexpr (NP >= 3) {
             pass_g<T, N, L, WG, S, fft_radix(N, 1), fft_ns(N, 1), MAP, true>(tw, ld, st);
@@ -160,7 +163,7 @@ __device__ __forceinline__ void tile_fft_io(cplx<T> *lds, const cplx<T> *__restr
             pass_g<T, N, L, WG, S, fft_radix(N, 2), fft_ns(N, 2), MAP, true>(tw, ld, st);
             __syncthreads();
         }
-        pass_g<T, N, L, WG, S, fft_radix(N, NP - 1), fft_ns(N, NP - 1), MAP, LAST_LDS>(tw, ld, storeLast);
+        pass_g<T, N, L, WG, S, fft_radix(N, NP - 1), fft_ns(N, NP - 1), MAPL, LAST_LDS>(tw, ld, storeLast);
     }
     if constexpr (LAST_LDS) __syncthreads();
 }
@@ -358,23 +361,120 @@ __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
     tile_fft_io<T, N, L, ES, WG, +1, 1, true, false>(lds, A.tw, lload, rstore);
 }
 
+// ---------------------------------------------------------------- C2C passes
+// PlanFFT<Imag> for two complex fields (LATfield2_PlanFFT_decl.hpp:608-719, :1059-1198): the
+// k-field holds mode (kx, ky, kz) at storage (d0, d1, d2) = (kz, kx, ky) (cKSite,
+// LATfield2_Site.hpp:219-220).  Three passes, each reading and writing every element once:
+//   x : rows along x, complex field <-> W[z][y][kx][c] (dense workspace);
+//   y : fft_strided_kernel along y in place on W;
+//   z : columns (kx, c) of W at one ky, transformed along z and written with kz fastest
+//       into the k-field (forward), or read with kz fastest and written into W (backward).
+template <typename T> struct C2CArgs {
+    const cplx<T> *in;
+    cplx<T> *out;
+    int C;
+    int rh;                 // ghost width of the complex position-space field
+    int64_t k_col0;         // k-field: offset of (kz, kx, ky, c) = 0
+    int64_t k_s1, k_s2;     // k-field strides of kx (storage dim 1) and ky (storage dim 2)
+    const cplx<T> *tw;
+};
+
+// x pass: block = P rows x C components = P*C lines along x; row r = y + N z.  DIR 0
+// (forward): field -> W; DIR 1 (backward): W -> field.
+template <typename T, int N, int L, int WG, int S, int DIR>
+__global__ __launch_bounds__(WG) void fft_x_c2c_kernel(C2CArgs<T> A) {
+    using V = cplx<T>;
+    constexpr int ES = L + 1;
+    extern __shared__ unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    const int C = A.C;
+    const int P = L / C;
+    const int64_t row0 = (int64_t)blockIdx.x * P;
+    const int64_t nrows = (int64_t)N * N;
+    const int rh = A.rh;
+    const int64_t G = N + 2 * rh;
+    // element 0 of line l in the field and in W, or -1 past the last row
+    auto field_base = [&](int l) -> int64_t {
+        const int p = l / C, c = l - p * C;
+        const int64_t r = row0 + p;
+        if (p >= P || r >= nrows) return -1;
+        const int y = (int)(r % N), z = (int)(r / N);
+        return ((int64_t)rh + G * ((int64_t)(y + rh) + G * (z + rh))) * C + c;
+    };
+    auto work_base = [&](int l) -> int64_t {
+        const int p = l / C, c = l - p * C;
+        const int64_t r = row0 + p;
+        if (p >= P || r >= nrows) return -1;
+        return r * N * C + c;
+    };
+    auto gload = [&](int l, int e) {
+        const int64_t b = DIR == 0 ? field_base(l) : work_base(l);
+        return b < 0 ? mk<T>(0, 0) : A.in[b + (int64_t)e * C];
+    };
+    auto gstore = [&](int l, int e, V v) {
+        const int64_t b = DIR == 0 ? work_base(l) : field_base(l);
+        if (b >= 0) A.out[b + (int64_t)e * C] = v;
+    };
+    tile_fft_io<T, N, L, ES, WG, S, 1, false, false>(lds, A.tw, gload, gstore);
+}
+
+// z pass: tile = L consecutive columns (kx, c) of W at ky = blockIdx.y, N elements along z.
+// The W side is read or written line-fastest (coalesced across the columns), the k-field side
+// element-fastest (each column's N values along kz are consecutive, stride C); the first and
+// last butterfly passes do the HBM accesses with those two thread maps.
+template <typename T, int N, int L, int WG, int S, int DIR>
+__global__ __launch_bounds__(WG) void fft_z_c2c_kernel(C2CArgs<T> A) {
+    using V = cplx<T>;
+    constexpr int ES = L + 1;
+    extern __shared__ unsigned char smem_raw[];
+    V *lds = reinterpret_cast<V *>(smem_raw);
+    const int C = A.C;
+    const int NC = N * C;
+    const int col0 = blockIdx.x * L;
+    const int ky = blockIdx.y;
+    const int ncols = min(L, NC - col0);
+    const int64_t wbase = (int64_t)ky * NC + col0;  // W[z][ky][col]: element z at + z N^2 C
+    const int64_t wes = (int64_t)N * NC;
+    auto kaddr = [&](int l, int e) {
+        const int q = col0 + l, kx = q / C, c = q - kx * C;
+        return A.k_col0 + (int64_t)e * C + (int64_t)kx * A.k_s1 + (int64_t)ky * A.k_s2 + c;
+    };
+    if constexpr (DIR == 0) {
+        auto wload = [&](int l, int e) { return l < ncols ? A.in[wbase + (int64_t)e * wes + l] : mk<T>(0, 0); };
+        auto kstore = [&](int l, int e, V v) {
+            if (l < ncols) A.out[kaddr(l, e)] = v;
+        };
+        tile_fft_io<T, N, L, ES, WG, S, 1, false, false, 0, 1>(lds, A.tw, wload, kstore);
+    } else {
+        auto kload = [&](int l, int e) { return l < ncols ? A.in[kaddr(l, e)] : mk<T>(0, 0); };
+        auto wstore = [&](int l, int e, V v) {
+            if (l < ncols) A.out[wbase + (int64_t)e * wes + l] = v;
+        };
+        tile_fft_io<T, N, L, ES, WG, S, 1, false, false, 1, 0>(lds, A.tw, kload, wstore);
+    }
+}
+
 // ---------------------------------------------------------------- launch configuration
 template <typename T> struct Tune;
 template <> struct Tune<float> {
     static constexpr int WG = 256;
-    template <int N> static constexpr int strided_L() { return N >= 1024 ? 8 : 16; }
+    // 16 columns (128-byte rows); at N = 1024 the tile is 128 KB of the CU's 160 KB of LDS
+    template <int N> static constexpr int strided_L() { return 16; }
     template <int N> static constexpr int green_L() { return N >= 512 ? 8 : 16; }
     // x pass: lines per tile for one component (row pairs), and the widest tile, used when
     // the field has more components than the narrow tile holds
     template <int N> static constexpr int x_L() { return N >= 512 ? 4 : (N >= 128 ? 12 : 24); }
     template <int N> static constexpr int x_L_max() { return N >= 512 ? 6 : (N >= 128 ? 12 : 24); }
+    // C2C z pass: 128-byte column reads (padded LDS tile N x (L+1), up to 136 KB at 1024)
+    template <int N> static constexpr int c2c_z_L() { return 16; }
 };
 template <> struct Tune<double> {
     static constexpr int WG = 256;
-    template <int N> static constexpr int strided_L() { return N >= 1024 ? 4 : 8; }
+    template <int N> static constexpr int strided_L() { return 8; }
     template <int N> static constexpr int green_L() { return N >= 1024 ? 4 : 8; }
     template <int N> static constexpr int x_L() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
     template <int N> static constexpr int x_L_max() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
+    template <int N> static constexpr int c2c_z_L() { return 8; }
 };
 
 }  // namespace lf2mi
@@ -382,11 +482,13 @@ template <> struct Tune<double> {
 // ---------------------------------------------------------------- plan object
 struct lf2mi_fft_plan_s {
     int N, C, rh, kh, dtype;
+    int c2c;         // 1: complex position-space field (PlanFFT<Imag> of two Field<Imag>)
     int device;
     void *tw;        // N complex twiddles (dtype)
     double *green_s; // N doubles
     float *green_sf; // N floats
-    void *work;      // backward / Poisson workspace: N * N rows of wcols complex
+    void *work;      // R2C backward / Poisson workspace: N * N rows of wcols complex; C2C:
+                     // W[z][y][kx][c], N^3 C complex
     int64_t wcols;   // padded columns per workspace row
     int64_t bytes;
 };
@@ -441,6 +543,36 @@ static int launch_x(const XArgs<T> &A, int S, hipStream_t st) {
     return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", A.C, LMAX);
 }
 
+template <typename T, int N>
+static int launch_c2c(const C2CArgs<T> &A, int pass, int S, hipStream_t st) {
+    constexpr int WG = Tune<T>::WG;
+    if (pass == 0) {
+        constexpr int L = Tune<T>::template x_L_max<N>();
+        if (A.C > L) return fail(LF2MI_EUNSUPPORTED, "FFT with %d components exceeds the x-pass tile (%d)", A.C, L);
+        const int P = L / A.C;
+        const unsigned grid = (unsigned)(((int64_t)N * N + P - 1) / P);
+        const size_t shm = sizeof(cplx<T>) * N * (L + 1);
+        static_assert(sizeof(cplx<T>) * N * (L + 1) <= 65536, "x tile within 64 KB of LDS");
+        LF2MI_PROF(S < 0 ? "fft_x_c2c_fwd" : "fft_x_c2c_inv", st);
+        if (S < 0)
+            fft_x_c2c_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
+        else
+            fft_x_c2c_kernel<T, N, L, WG, +1, 1><<<grid, WG, shm, st>>>(A);
+    } else {
+        constexpr int L = Tune<T>::template c2c_z_L<N>();
+        static_assert(sizeof(cplx<T>) * N * (L + 1) <= 163840, "z tile within the 160 KB of LDS of a CU");
+        dim3 grid((N * A.C + L - 1) / L, N);
+        const size_t shm = sizeof(cplx<T>) * N * (L + 1);
+        LF2MI_PROF(S < 0 ? "fft_z_c2c_fwd" : "fft_z_c2c_inv", st);
+        if (S < 0)
+            fft_z_c2c_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
+        else
+            fft_z_c2c_kernel<T, N, L, WG, +1, 1><<<grid, WG, shm, st>>>(A);
+    }
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
 #define LF2MI_DISPATCH_N(N_, CALL)                              \
     switch (N_) {                                                \
         case 8: { constexpr int NN = 8; return CALL; }           \
@@ -459,6 +591,9 @@ template <typename T> static int strided(int N, const StridedArgs<T> &A, int S, 
 }
 template <typename T> static int xpass(int N, const XArgs<T> &A, int S, hipStream_t st) {
     LF2MI_DISPATCH_N(N, (launch_x<T, NN>(A, S, st)));
+}
+template <typename T> static int c2cpass(int N, const C2CArgs<T> &A, int pass, int S, hipStream_t st) {
+    LF2MI_DISPATCH_N(N, (launch_c2c<T, NN>(A, pass, S, st)));
 }
 
 // k-field geometry
@@ -556,6 +691,64 @@ static int backward(lf2mi_fft_plan_s *p, const cplx<T> *kin, KGeo kg, void *rfie
     return xpass<T>(N, X, +1, st);
 }
 
+// C2C: k-field strides of the cKSite layout (kz fastest, then kx, then ky)
+template <typename T>
+static C2CArgs<T> c2c_args(const lf2mi_fft_plan_s *p) {
+    C2CArgs<T> A{};
+    const int64_t KG = p->N + 2 * p->kh;
+    A.C = p->C;
+    A.rh = p->rh;
+    A.k_s1 = KG * p->C;
+    A.k_s2 = KG * KG * p->C;
+    A.k_col0 = (int64_t)p->kh * (p->C + A.k_s1 + A.k_s2);
+    A.tw = (const cplx<T> *)p->tw;
+    return A;
+}
+
+template <typename T>
+static int c2c_forward(lf2mi_fft_plan_s *p, const cplx<T> *rfield, cplx<T> *kfield, hipStream_t st) {
+    const int N = p->N;
+    cplx<T> *W = (cplx<T> *)p->work;
+    C2CArgs<T> A = c2c_args<T>(p);
+    A.in = rfield;
+    A.out = W;
+    if (int rc = c2cpass<T>(N, A, 0, -1, st)) return rc;
+    StridedArgs<T> Y{};
+    Y.in = W;
+    Y.out = W;
+    Y.ncols = N * p->C;
+    Y.in_es = Y.out_es = (int64_t)N * p->C;
+    Y.in_bs = Y.out_bs = (int64_t)N * N * p->C;
+    Y.tw = A.tw;
+    Y.C = p->C;
+    if (int rc = strided<T>(N, Y, -1, 0, N, st)) return rc;
+    A.in = W;
+    A.out = kfield;
+    return c2cpass<T>(N, A, 1, -1, st);
+}
+
+template <typename T>
+static int c2c_backward(lf2mi_fft_plan_s *p, const cplx<T> *kfield, cplx<T> *rfield, hipStream_t st) {
+    const int N = p->N;
+    cplx<T> *W = (cplx<T> *)p->work;
+    C2CArgs<T> A = c2c_args<T>(p);
+    A.in = kfield;
+    A.out = W;
+    if (int rc = c2cpass<T>(N, A, 1, +1, st)) return rc;
+    StridedArgs<T> Y{};
+    Y.in = W;
+    Y.out = W;
+    Y.ncols = N * p->C;
+    Y.in_es = Y.out_es = (int64_t)N * p->C;
+    Y.in_bs = Y.out_bs = (int64_t)N * N * p->C;
+    Y.tw = A.tw;
+    Y.C = p->C;
+    if (int rc = strided<T>(N, Y, +1, 0, N, st)) return rc;
+    A.in = W;
+    A.out = rfield;
+    return c2cpass<T>(N, A, 0, +1, st);
+}
+
 template <typename T>
 static int poisson(lf2mi_fft_plan_s *p, const void *rho, void *phi, double shift, double scale, hipStream_t st) {
     const int N = p->N;
@@ -636,7 +829,7 @@ using namespace lf2mi;
 
 extern "C" {
 
-int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan) {
+static int plan_create(int N, int C, int rhalo, int khalo, int dtype, int c2c, lf2mi_fft_plan *plan) {
     if (!plan) return fail(LF2MI_EINVAL, "lf2mi_fft_plan_create: null plan pointer");
     *plan = nullptr;
     if (!is_pow2(N) || N < 8 || N > 1024) return fail(LF2MI_ERANGE, "FFT size N=%d: a power of two in [8, 1024] is required", N);
@@ -651,6 +844,7 @@ int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_f
     p->rh = rhalo;
     p->kh = khalo;
     p->dtype = dtype;
+    p->c2c = c2c;
     hipGetDevice(&p->device);
     const size_t esz = dtype == LF2MI_F32 ? sizeof(float2) : sizeof(double2);
     // twiddles and Green eigenvalues in FP64
@@ -671,7 +865,7 @@ int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_f
     }
     // workspace rows padded to a multiple of 16 complex values (128 B for float2)
     p->wcols = (((int64_t)(N / 2 + 1) * C + 15) / 16) * 16;
-    const int64_t wbytes = (int64_t)esz * p->wcols * N * N;
+    const int64_t wbytes = c2c ? (int64_t)esz * N * N * N * C : (int64_t)esz * p->wcols * N * N;
     int rc = LF2MI_OK;
     hipError_t e;
     if ((e = hipMalloc(&p->tw, esz * N)) != hipSuccess) rc = hip_fail(e, "hipMalloc(twiddles)", __FILE__, __LINE__);
@@ -708,6 +902,14 @@ int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_f
     return LF2MI_OK;
 }
 
+int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan) {
+    return plan_create(N, C, rhalo, khalo, dtype, 0, plan);
+}
+
+int lf2mi_fft_plan_create_c2c(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan) {
+    return plan_create(N, C, rhalo, khalo, dtype, 1, plan);
+}
+
 int lf2mi_fft_plan_destroy(lf2mi_fft_plan p) {
     if (!p) return LF2MI_OK;
     hipFree(p->tw);
@@ -729,6 +931,14 @@ int lf2mi_fft_execute(lf2mi_fft_plan p, int direction, void *rfield, void *kfiel
     if (direction != LF2MI_FFT_FORWARD && direction != LF2MI_FFT_BACKWARD) return LF2MI_OK;
     if (!rfield || !kfield) return fail(LF2MI_EINVAL, "lf2mi_fft_execute: null field");
     hipStream_t st = as_stream(stream);
+    if (p->c2c) {
+        if (p->dtype == LF2MI_F32) {
+            if (direction == LF2MI_FFT_FORWARD) return c2c_forward<float>(p, (const float2 *)rfield, (float2 *)kfield, st);
+            return c2c_backward<float>(p, (const float2 *)kfield, (float2 *)rfield, st);
+        }
+        if (direction == LF2MI_FFT_FORWARD) return c2c_forward<double>(p, (const double2 *)rfield, (double2 *)kfield, st);
+        return c2c_backward<double>(p, (const double2 *)kfield, (double2 *)rfield, st);
+    }
     const KGeo kg = kfield_geo(p);
     if (p->dtype == LF2MI_F32) {
         if (direction == LF2MI_FFT_FORWARD) return forward<float>(p, rfield, (float2 *)kfield, kg, 0.0f, st);
@@ -769,7 +979,7 @@ int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, doubl
 
 int lf2mi_poisson_solve(lf2mi_fft_plan p, const void *rho, void *phi, double shift, double scale, void *stream) {
     if (!p || !rho || !phi) return fail(LF2MI_EINVAL, "lf2mi_poisson_solve: null argument");
-    if (p->C != 1) return fail(LF2MI_EINVAL, "lf2mi_poisson_solve needs a one-component plan");
+    if (p->C != 1 || p->c2c) return fail(LF2MI_EINVAL, "lf2mi_poisson_solve needs a one-component R2C plan");
     if (p->dtype == LF2MI_F32) return poisson<float>(p, rho, phi, shift, scale, as_stream(stream));
     return poisson<double>(p, rho, phi, shift, scale, as_stream(stream));
 }

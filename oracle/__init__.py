@@ -68,6 +68,9 @@ def lib():
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
             "orc_fft_r2c": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "orc_fft_c2r": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+            "orc_fft_c2c_forward": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+            "orc_fft_c2c_backward": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+            "orc_fft_c2c_backward_reference": [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             "orc_poisson_green": [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double],
             "orc_force_field": [_dp, _dp, ctypes.c_int, ctypes.c_int],
@@ -251,6 +254,38 @@ def fft_c2r(k, r, N, rh=1, kh=0):
     assert r.shape[-1] == C
     _check(lib().orc_fft_c2r(_p(k), _p(r), N, C, rh, kh), "orc_fft_c2r")
     return r
+
+
+def complex_field(N: int, h: int = 1, C: int = 1) -> np.ndarray:
+    """a Field<Imag> on an N^3 lattice (position space, or the C2C k-lattice)"""
+    G = N + 2 * h
+    return np.zeros((G, G, G, C), dtype=np.complex128)
+
+
+def fft_c2c_forward(f, N, rh=1, kh=0):
+    """C2C forward; returns the k-field [ky, kx, kz, c] (storage order (kz, kx, ky))"""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    C = f.shape[-1]
+    k = complex_field(N, kh, C)
+    _check(lib().orc_fft_c2c_forward(_p(f.view(np.float64)), _p(k.view(np.float64)), N, C, rh, kh),
+           "orc_fft_c2c_forward")
+    return k
+
+
+def fft_c2c_backward(k, f, N, rh=1, kh=0):
+    k = np.ascontiguousarray(k, dtype=np.complex128)
+    C = k.shape[-1]
+    _check(lib().orc_fft_c2c_backward(_p(k.view(np.float64)), _p(f.view(np.float64)), N, C, rh, kh),
+           "orc_fft_c2c_backward")
+    return f
+
+
+def fft_c2c_backward_reference(k, f, N, h=0):
+    k = np.ascontiguousarray(k, dtype=np.complex128)
+    C = k.shape[-1]
+    _check(lib().orc_fft_c2c_backward_reference(_p(k.view(np.float64)), _p(f.view(np.float64)), N, C, h),
+           "orc_fft_c2c_backward_reference")
+    return f
 
 
 def poisson_green(k, N, kh=0, res=None, scale=1.0):

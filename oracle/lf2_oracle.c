@@ -813,6 +813,151 @@ int orc_fft_c2r(const double *k, double *r, int N, int C, int rh, int kh) {
     return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * C2C PlanFFT (two Field<Imag>), single rank.  Complex fields are interleaved
+ * (re, im) doubles, C components per site.
+ * ------------------------------------------------------------------------- */
+
+/* one FFTW-style "many" 1-D transform: `howmany` lines of n points, element e of line
+ * m at in[idist m + istride e], written to out[odist m + ostride e] (complex units) */
+static void many_dft(const fft_plan1 *P, const double *in, i64 istride, i64 idist, double *out, i64 ostride,
+                     i64 odist, i64 howmany, int sign) {
+    const int n = P->n;
+    double *line = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+    double *tmp = line + 2 * n;
+    for (i64 m = 0; m < howmany; ++m) {
+        for (int e = 0; e < n; ++e) {
+            line[2 * e] = in[2 * (idist * m + istride * e)];
+            line[2 * e + 1] = in[2 * (idist * m + istride * e) + 1];
+        }
+        fft1(P, line, sign, tmp);
+        for (int e = 0; e < n; ++e) {
+            out[2 * (odist * m + ostride * e)] = line[2 * e];
+            out[2 * (odist * m + ostride * e) + 1] = line[2 * e + 1];
+        }
+    }
+    free(line);
+}
+
+/* transpose_0_2 / transpose_1_2 (LATfield2_PlanFFT_decl.hpp:1396-1466) */
+static void transpose_0_2(const double *in, double *out, i64 di, i64 dj, i64 dk) {
+    for (i64 i = 0; i < di; ++i)
+        for (i64 j = 0; j < dj; ++j)
+            for (i64 k = 0; k < dk; ++k) {
+                out[2 * (k + dk * (j + i * dj))] = in[2 * (i + di * (j + k * dj))];
+                out[2 * (k + dk * (j + i * dj)) + 1] = in[2 * (i + di * (j + k * dj)) + 1];
+            }
+}
+static void transpose_1_2(const double *in, double *out, i64 di, i64 dj, i64 dk) {
+    for (i64 i = 0; i < di; ++i)
+        for (i64 j = 0; j < dj; ++j)
+            for (i64 k = 0; k < dk; ++k) {
+                out[2 * (i + di * (k + j * dk))] = in[2 * (i + di * (j + k * dj))];
+                out[2 * (i + di * (k + j * dk)) + 1] = in[2 * (i + di * (j + k * dj)) + 1];
+            }
+}
+
+/* PlanFFT::execute(FFT_FORWARD), C2C (LATfield2_PlanFFT_decl.hpp:1061-1133), the steps of
+ * one rank: fPlan_i_ along x into temp [x][z][y], transpose_0_2, fPlan_j_ along the slowest
+ * axis (y), transpose_1_2, fPlan_k_ along z into the k-field with kz on storage dim 0, kx
+ * on dim 1, ky on dim 2 (plans at :678-680).  The reference steps the k-field with the
+ * position lattice's jumps (rJump_); the k lattice's own jumps are used here, which is the
+ * layout cKSite reads (LATfield2_Site.hpp:219-220) and the same when rh == kh. */
+int orc_fft_c2c_forward(const double *f, double *k, int N, int C, int rh, int kh) {
+    if (!f || !k || N < 1 || C < 1 || rh < 0 || kh < 0) return ORC_EBADARG;
+    const i64 n3 = (i64)N * N * N, N2 = (i64)N * N;
+    double *temp = (double *)malloc(sizeof(double) * 2 * (size_t)n3);
+    double *temp1 = (double *)malloc(sizeof(double) * 2 * (size_t)n3);
+    fft_plan1 P;
+    if (!temp || !temp1 || plan1_init(&P, N)) { free(temp); free(temp1); return ORC_ENOMEM; }
+    const i64 G = (i64)N + 2 * rh, KG = (i64)N + 2 * kh;
+    const double *cData = f + 2 * (i64)rh * (1 + G + G * G) * C;
+    double *kData = k + 2 * (i64)kh * (1 + KG + KG * KG) * C;
+    for (int comp = 0; comp < C; ++comp) {
+        for (int l = 0; l < N; ++l)
+            many_dft(&P, cData + 2 * (G * G * l * C + comp), C, G * C, temp + 2 * ((i64)l * N), N2, 1, N, -1);
+        memcpy(temp1, temp, sizeof(double) * 2 * (size_t)n3);
+        transpose_0_2(temp1, temp, N, N, N);
+        many_dft(&P, temp, N2, 1, temp, N2, 1, N2, -1);
+        memcpy(temp1, temp, sizeof(double) * 2 * (size_t)n3);
+        transpose_1_2(temp1, temp, N, N, N);
+        for (int l = 0; l < N; ++l)
+            many_dft(&P, temp + 2 * ((i64)l * N), N2, 1, kData + 2 * (KG * KG * l * C + comp), C, KG * C, N, -1);
+    }
+    plan1_free(&P);
+    free(temp);
+    free(temp1);
+    return 0;
+}
+
+/* The inverse of orc_fft_c2c_forward: the unnormalised e^{+2 pi i k.x/N} DFT from the
+ * k-field (mode (kx, ky, kz) at storage (kz, kx, ky)) into the complex interior, so
+ * backward(forward(f)) = N^3 f.  This is the published inverse DFT, not the reference's
+ * backward (see orc_fft_c2c_backward_reference). */
+int orc_fft_c2c_backward(const double *k, double *f, int N, int C, int rh, int kh) {
+    if (!f || !k || N < 1 || C < 1 || rh < 0 || kh < 0) return ORC_EBADARG;
+    const i64 n3 = (i64)N * N * N;
+    double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)n3);
+    fft_plan1 P;
+    if (!buf || plan1_init(&P, N)) { free(buf); return ORC_ENOMEM; }
+    const i64 geo[2] = {N, N};
+    for (int c = 0; c < C; ++c) {
+        /* buf[kx + N (ky + N kz)] */
+        for (int ky = 0; ky < N; ++ky)
+            for (int kx = 0; kx < N; ++kx)
+                for (int kz = 0; kz < N; ++kz) {
+                    const i64 b = kx + (i64)N * (ky + (i64)N * kz);
+                    const i64 o = sidx(N, kh, kz, kx, ky) * C + c;
+                    buf[2 * b] = k[2 * o];
+                    buf[2 * b + 1] = k[2 * o + 1];
+                }
+        fft_lines(&P, buf, (i64)N * N, base_ax0, geo, 1, +1);
+        fft_lines(&P, buf, (i64)N * N, base_ax1, geo, N, +1);
+        fft_lines(&P, buf, (i64)N * N, base_ax2, geo, (i64)N * N, +1);
+        for (int z = 0; z < N; ++z)
+            for (int y = 0; y < N; ++y)
+                for (int x = 0; x < N; ++x) {
+                    const i64 b = x + (i64)N * (y + (i64)N * z);
+                    const i64 o = sidx(N, rh, x, y, z) * C + c;
+                    f[2 * o] = buf[2 * b];
+                    f[2 * o + 1] = buf[2 * b + 1];
+                }
+    }
+    plan1_free(&P);
+    free(buf);
+    return 0;
+}
+
+/* PlanFFT::execute(FFT_BACKWARD), C2C, as the reference runs it on one rank
+ * (LATfield2_PlanFFT_decl.hpp:1135-1196), with one ghost width h for both fields:
+ * bPlan_k_ along kz into temp [z][ky][kx]; bPlan_j_ along the slowest axis, which is z
+ * again because the transposition before it is commented out (:1165-1167); transpose_0_2;
+ * bPlan_i_ along kx into position storage (x, l, m) <- temp line (l, m) = (z, ky). */
+int orc_fft_c2c_backward_reference(const double *k, double *f, int N, int C, int h) {
+    if (!f || !k || N < 1 || C < 1 || h < 0) return ORC_EBADARG;
+    const i64 n3 = (i64)N * N * N, N2 = (i64)N * N;
+    double *temp = (double *)malloc(sizeof(double) * 2 * (size_t)n3);
+    double *temp1 = (double *)malloc(sizeof(double) * 2 * (size_t)n3);
+    fft_plan1 P;
+    if (!temp || !temp1 || plan1_init(&P, N)) { free(temp); free(temp1); return ORC_ENOMEM; }
+    const i64 G = (i64)N + 2 * h;
+    const double *kData = k + 2 * (i64)h * (1 + G + G * G) * C;
+    double *cData = f + 2 * (i64)h * (1 + G + G * G) * C;
+    for (int comp = 0; comp < C; ++comp) {
+        for (int l = 0; l < N; ++l)
+            many_dft(&P, kData + 2 * (G * G * l * C + comp), C, G * C, temp + 2 * ((i64)l * N), N2, 1, N, +1);
+        many_dft(&P, temp, N2, 1, temp, N2, 1, N2, +1);
+        memcpy(temp1, temp, sizeof(double) * 2 * (size_t)n3);
+        transpose_0_2(temp1, temp, N, N, N);
+        for (int l = 0; l < N; ++l)
+            many_dft(&P, temp + 2 * ((i64)l * N), N2, 1, cData + 2 * (G * G * l * C + comp), C, G * C, N, +1);
+    }
+    plan1_free(&P);
+    free(temp);
+    free(temp1);
+    return 0;
+}
+
 /* examples/poissonSolver.cpp:99-111 (zero mode :100-104), k coordinates in the rKSite
  * sense: kx = lattice dim 0, ky = dim 2, kz = dim 1 (LATfield2_Site.hpp:319-320). */
 int orc_poisson_green(double *k, int N, int C, int kh, double res, double scale) {

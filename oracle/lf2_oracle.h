@@ -100,6 +100,17 @@ int orc_brute_deposit(const double *pos, const double *vel, const double *mass,
 int orc_fft_r2c(const double *r, double *k, int N, int C, int rh, int kh);
 int orc_fft_c2r(const double *k, double *r, int N, int C, int rh, int kh);
 
+/* C2C PlanFFT of two Field<Imag> (LATfield2_PlanFFT_decl.hpp:608-719, :1059-1198).
+ * Complex fields: interleaved (re, im) doubles, C components per site.
+ * forward : unnormalised e^{-} DFT of the complex interior; mode (kx, ky, kz) at k-lattice
+ *           storage (kz, kx, ky) (cKSite, LATfield2_Site.hpp:219-220); input kept;
+ * backward: the unnormalised e^{+} inverse, backward(forward(f)) = N^3 f;
+ * backward_reference: what the reference's backward computes on one rank (its z pass
+ *           runs twice and its y pass not at all), with one ghost width h for both. */
+int orc_fft_c2c_forward(const double *f, double *k, int N, int C, int rh, int kh);
+int orc_fft_c2c_backward(const double *k, double *f, int N, int C, int rh, int kh);
+int orc_fft_c2c_backward_reference(const double *k, double *f, int N, int C, int h);
+
 /* Green's function multiply of examples/poissonSolver.cpp:99-111:
  *   phi_k = rho_k * scale / lambda,  lambda = 2 (sum_i cos(2 pi k_i/N) - 3) / res^2,
  *   zero mode set to 0.  In place on a k-field in the layout above. */

@@ -1,4 +1,4 @@
-"""GPU parity: ghost fill, CIC folds, R2C/C2R FFT, Green's function, fused Poisson solve.
+"""GPU parity: ghost fill, CIC folds, R2C/C2R and C2C FFT, Green's function, fused Poisson solve.
 
 Oracle: oracle/ (FP64 restatement, pinned in tests/test_oracle.py).  Bars:
   * ghost fill and folds are copies / single adds: bit-exact in both precisions;
@@ -111,6 +111,91 @@ def test_fft_backward_matches_oracle(prec, tol, N, h, kh, C):
         assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
     assert np.array_equal(kfield_np(k), kin)            # k-field preserved
     assert np.all(got[:h] == 0) and np.all(got[:, :, -h:] == 0)  # real ghosts untouched
+
+
+# ---------------------------------------------------------------- C2C PlanFFT
+def _c2c_case(N, h, kh, C, prec, seed):
+    r = lf.CField(lf.Lattice(3, N, h), C, DT[prec])
+    k = lf.CField(lf.Lattice(3, N, kh), C, DT[prec])
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    r.t.copy_(torch.randn(r.t.shape, generator=g, dtype=torch.float64).to(r.t.dtype))
+    return r, k, lf.PlanFFT(r, k)
+
+
+def _close(got, ref, prec, tol):
+    if prec == "f32":
+        return rel_l2(got, ref) <= tol
+    return np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+@pytest.mark.parametrize("N,h,kh,C", [(8, 1, 1, 1), (16, 0, 1, 1), (32, 2, 1, 3), (64, 1, 0, 1), (128, 1, 1, 1),
+                                      (16, 1, 2, 6)])
+def test_c2c_forward_matches_oracle(prec, tol, N, h, kh, C):
+    """PlanFFT<Imag>(Field<Imag>*, Field<Imag>*) forward (LATfield2_PlanFFT_decl.hpp:1061-1133)"""
+    announce(f"c2c forward {prec} N={N} h={h} kh={kh} C={C}")
+    r, k, plan = _c2c_case(N, h, kh, C, prec, seed=3 * N + C)
+    rin = kfield_np(r).copy()
+    plan.execute(lf.FFT_FORWARD)
+    torch.cuda.synchronize()
+    ref = orc.fft_c2c_forward(rin, N, h, kh)
+    got = kfield_np(k)
+    assert _close(got, ref, prec, tol), rel_l2(got, ref)
+    assert np.array_equal(kfield_np(r), rin)  # input preserved
+    if kh:
+        assert np.all(got[:kh] == 0) and np.all(got[:, :, :kh] == 0)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+@pytest.mark.parametrize("N,h,kh,C", [(8, 1, 0, 1), (32, 1, 1, 2), (64, 2, 1, 1), (256, 1, 1, 1)])
+def test_c2c_backward_matches_oracle_inverse(prec, tol, N, h, kh, C):
+    announce(f"c2c backward {prec} N={N} h={h} kh={kh} C={C}")
+    r = lf.CField(lf.Lattice(3, N, h), C, DT[prec])
+    k = lf.CField(lf.Lattice(3, N, kh), C, DT[prec])
+    g = torch.Generator(device="cpu").manual_seed(N + 11 * C)
+    k.t.copy_(torch.randn(k.t.shape, generator=g, dtype=torch.float64).to(k.t.dtype))
+    plan = lf.PlanFFT(r, k)
+    kin = kfield_np(k).copy()
+    plan.execute(lf.FFT_BACKWARD)
+    torch.cuda.synchronize()
+    ref = orc.complex_field(N, h, C)
+    orc.fft_c2c_backward(np.ascontiguousarray(kin), ref, N, h, kh)
+    got = kfield_np(r)
+    assert _close(got, ref, prec, tol), rel_l2(got, ref)
+    assert np.array_equal(kfield_np(k), kin)  # k-field preserved
+    if h:
+        assert np.all(got[:h] == 0) and np.all(got[:, :, -h:] == 0)  # position ghosts untouched
+
+
+@pytest.mark.parametrize("N", [512, 1024])
+def test_c2c_full_size_round_trip_and_plane_wave_fp32(N):
+    """size-independent properties at 512^3 / 1024^3: one plane wave -> N^3 at its cKSite,
+    zeros elsewhere; backward(forward(f)) = N^3 f (both on the device)"""
+    announce(f"c2c full size fp32 N={N}")
+    lat = lf.Lattice(3, N, 1)
+    r, k = lf.CField(lat, 1, lf.F32), lf.CField(lf.Lattice(3, N, 0), 1, lf.F32)
+    plan = lf.PlanFFT(r, k)
+    a, b, c = 3, N - 7, 5
+    ar = torch.arange(N, device="cuda", dtype=torch.float64)
+    ph = 2 * np.pi * (c * ar[:, None, None] + b * ar[None, :, None] + a * ar[None, None, :]) / N
+    r.interior()[..., 0, 0] = torch.cos(ph).float()
+    r.interior()[..., 0, 1] = torch.sin(ph).float()
+    del ph
+    plan.execute(lf.FFT_FORWARD)
+    kc = k.as_complex()[..., 0]  # [ky, kx, kz]
+    peak = kc[b, a, c].clone()
+    kc[b, a, c] = 0
+    off = torch.linalg.vector_norm(kc).item() / N ** 3
+    assert abs(peak.real.item() / N ** 3 - 1) < 1e-5 and abs(peak.imag.item()) / N ** 3 < 1e-5
+    assert off < 5e-6, off  # L2 of the other modes, relative to the peak
+    g = torch.Generator(device="cuda").manual_seed(N)
+    r.t.copy_(torch.randn(r.t.shape, generator=g, device="cuda", dtype=torch.float32))
+    f0 = r.interior().clone()
+    plan.execute(lf.FFT_FORWARD)
+    plan.execute(lf.FFT_BACKWARD)
+    torch.cuda.synchronize()
+    err = torch.linalg.vector_norm(r.interior() / N ** 3 - f0) / torch.linalg.vector_norm(f0)
+    assert err.item() <= 2e-6, err.item()
 
 
 def test_wave_test_kats_on_device_fp64():

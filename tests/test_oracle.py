@@ -153,6 +153,93 @@ def test_fft_roundtrip_is_n3_identity():
     assert np.all(r2[0] == 0.0) and np.all(r2[:, :, -1] == 0.0)
 
 
+# --------------------------------------------------------------------------- C2C PlanFFT
+def _cinterior(f, h):
+    G = f.shape[0]
+    return f[h:G - h, h:G - h, h:G - h]
+
+
+@pytest.mark.parametrize("N,C,rh,kh", [(8, 1, 1, 1), (8, 2, 1, 2), (6, 1, 0, 1), (16, 3, 2, 0)])
+def test_c2c_forward_matches_numpy_fftn_in_cksite_layout(N, C, rh, kh):
+    """fPlan_i_/j_/k_ and both transposes (LATfield2_PlanFFT_decl.hpp:1061-1133) land mode
+    (kx, ky, kz) at storage (kz, kx, ky), the cKSite map {1, 2, 0} (LATfield2_Site.hpp:219)"""
+    rng = np.random.default_rng(N + 10 * C)
+    f = orc.complex_field(N, rh, C)
+    _cinterior(f, rh)[...] = rng.standard_normal((N, N, N, C)) + 1j * rng.standard_normal((N, N, N, C))
+    f0 = f.copy()
+    k = orc.fft_c2c_forward(f, N, rh, kh)
+    F = np.fft.fftn(_cinterior(f, rh), axes=(0, 1, 2))      # [kz, ky, kx, c]
+    assert np.allclose(_cinterior(k, kh), np.transpose(F, (1, 2, 0, 3)), rtol=1e-12, atol=1e-9)
+    assert np.array_equal(f, f0)  # FFTW_PRESERVE_INPUT
+    if kh:
+        assert np.all(k[0] == 0) and np.all(k[:, :, -1] == 0)
+
+
+def test_c2c_plane_wave_closed_form():
+    N, h = 8, 1
+    a, b, c = 3, 5, 1
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    f = orc.complex_field(N, h, 1)
+    _cinterior(f, h)[..., 0] = np.exp(2j * np.pi * (a * x + b * y + c * z) / N)
+    k = orc.fft_c2c_forward(f, N, h, h)
+    want = np.zeros((N, N, N), dtype=np.complex128)
+    want[b, a, c] = N ** 3  # array [d2, d1, d0] = [ky, kx, kz]
+    assert np.allclose(_cinterior(k, h)[..., 0], want, atol=1e-9)
+
+
+def test_c2c_backward_is_the_inverse():
+    N, C, rh, kh = 12, 2, 2, 1  # 12: the direct-DFT branch
+    rng = np.random.default_rng(5)
+    f = orc.complex_field(N, rh, C)
+    _cinterior(f, rh)[...] = rng.standard_normal((N, N, N, C)) + 1j * rng.standard_normal((N, N, N, C))
+    k = orc.fft_c2c_forward(f, N, rh, kh)
+    k0 = k.copy()
+    g = orc.complex_field(N, rh, C)
+    orc.fft_c2c_backward(k, g, N, rh, kh)
+    assert np.allclose(_cinterior(g, rh) / N ** 3, _cinterior(f, rh), atol=1e-12)
+    assert np.array_equal(k, k0)
+    assert np.all(g[0] == 0) and np.all(g[:, :, -1] == 0)
+
+
+def test_c2c_reference_backward_reproduces_the_example_normalisation():
+    """examples/fft_c2c_r2c_sametime.cpp:102-123 (BoxSize 16, one component): rho = 1,
+    forward then backward, and the example prints sum(Re rho) / N^3 / N^2 = 1 -- the
+    reference's backward skips the transposition before its second pass (PlanFFT_decl.hpp:
+    1165-1167), so it runs the z transform twice and never the y one.  The true inverse
+    gives N (every site N^3)."""
+    N = 16
+    f = orc.complex_field(N, 0, 1)
+    f[...] = 1.0
+    k = orc.fft_c2c_forward(f, N, 0, 0)
+    ref = orc.complex_field(N, 0, 1)
+    orc.fft_c2c_backward_reference(k, ref, N, 0)
+    assert math.isclose(ref.real.sum() / N ** 3 / N ** 2, 1.0, rel_tol=1e-12)
+    # the whole result sits on the N sites with storage dims 1 and 2 at 0, each N^4
+    want = np.zeros_like(ref)
+    want[0, 0, :, 0] = N ** 4
+    assert np.allclose(ref, want, atol=1e-6)
+    inv = orc.complex_field(N, 0, 1)
+    orc.fft_c2c_backward(k, inv, N, 0, 0)
+    assert math.isclose(inv.real.sum() / N ** 3 / N ** 2, N, rel_tol=1e-12)
+    # on a generic input the reference's backward is not an inverse
+    rng = np.random.default_rng(1)
+    N = 8
+    f = orc.complex_field(N, 1, 1)
+    _cinterior(f, 1)[...] = rng.standard_normal((N, N, N, 1)) + 1j * rng.standard_normal((N, N, N, 1))
+    k = orc.fft_c2c_forward(f, N, 1, 1)
+    ref = orc.complex_field(N, 1, 1)
+    orc.fft_c2c_backward_reference(k, ref, N, 1)
+    assert not np.allclose(_cinterior(ref, 1) / N ** 3, _cinterior(f, 1), atol=1e-3)
+    # closed form of what it computes, at storage (d0, d1, d2) = (x, w, ky):
+    # N^2 sum_{y,z} f(x, y, z) e^{-2 pi i ky y / N} e^{+2 pi i w z / N}
+    fi = _cinterior(f, 1)[..., 0]                                  # [z, y, x]
+    n = np.arange(N)
+    Ey = np.exp(-2j * np.pi * np.outer(n, n) / N)                  # [y, ky]
+    Ez = np.exp(+2j * np.pi * np.outer(n, n) / N)                  # [z, w]
+    want = N ** 2 * np.einsum("zyx,yk,zw->kwx", fi, Ey, Ez)        # [d2, d1, d0]
+    assert np.allclose(_cinterior(ref, 1)[..., 0], want, atol=1e-8)
+
+
 def test_fft_golden_fixture():
     g = np.load(os.path.join(GOLD, "fft_n8.npz"))
     N = int(g["N"])
