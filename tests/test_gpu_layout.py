@@ -31,7 +31,7 @@ def _upload(N, pos, vel, mass=None):
 
 
 def _check_state(parts, N, tag):
-    """download, counts, the three deposits and a gather against the oracle"""
+    """download, counts, the five deposits and a gather against the oracle"""
     d = _by_id(parts.download())
     pos = np.ascontiguousarray(d["pos"])
     assert np.array_equal(d["cell"], orc.bin_keys(pos, N)), tag
@@ -39,7 +39,9 @@ def _check_state(parts, N, tag):
     lat = lf.Lattice(3, N, 1)
     for kind, C, proj, comm in ((lf.PROJ_SCALAR, 1, orc.scalar_project, orc.scalar_comm),
                                 (lf.PROJ_VECTOR, 3, orc.vector_project, orc.vector_comm),
-                                (lf.PROJ_VECVEC, 6, orc.vecvec_project, orc.vecvec_comm)):
+                                (lf.PROJ_VECVEC, 6, orc.vecvec_project, orc.vecvec_comm),
+                                (lf.PROJ_VECTOR_NGP, 3, orc.vector_ngp_project, orc.vector_ngp_comm),
+                                (lf.PROJ_SYMTENSOR_NGP, 6, orc.symtensor_ngp_project, orc.symtensor_ngp_comm)):
         f = lf.Field(lat, C, lf.F32)
         f.zero()
         parts.deposit(kind, f, lf.DEP_PERIODIC)

@@ -162,6 +162,24 @@ def test_cic_ngp_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mo
             assert rel_l2(got[..., c], ref[..., c]) <= tol, (mode, f.C, c, rel_l2(got[..., c], ref[..., c]))
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_deposits_of_an_empty_container_add_nothing(prec):
+    announce(f"empty container deposits {prec}")
+    N = 16
+    parts = lf.Particles(N, 10, DT[prec])
+    lat = lf.Lattice(3, N, 1)
+    for kind, C in ((lf.PROJ_SCALAR, 1), (lf.PROJ_VECTOR, 3), (lf.PROJ_VECVEC, 6), (lf.PROJ_VECTOR_NGP, 3),
+                    (lf.PROJ_SYMTENSOR_NGP, 6)):
+        f = lf.Field(lat, C, DT[prec])
+        f.t.fill_(0.25)
+        for target in (lf.DEP_UNFOLDED, lf.DEP_PERIODIC):
+            parts.deposit(kind, f, target)
+        torch.cuda.synchronize()
+        assert bool((f.t == 0.25).all()), kind
+    with pytest.raises(lf.Lf2miError):
+        parts.deposit(5, lf.Field(lat, 6, DT[prec]), lf.DEP_PERIODIC)  # no such kind
+
+
 def test_fixed_point_deposit_is_reproducible_and_accurate():
     N = 32
     announce(f"fixed-point deposit N={N}")
