@@ -255,6 +255,14 @@ int lf2mi_parts_vector_from(lf2mi_parts p, int64_t offset, int real_dtype);
  * the histogram of Particles::getPartCoord keys) */
 int lf2mi_parts_cell_counts(lf2mi_parts p, int32_t *counts, void *stream);
 
+/* Particles::getPartCoord without a container (particles/LATfield2_Particles.hpp:464-467):
+ * for np FP64 positions (device, x,y,z per particle, box length 1) the key
+ * x + N (y + N z) with c = (int)floor(pos N) % N per axis, or -1 where a coordinate is
+ * negative (the particles addParticle_global does not keep); counts (N^3 int32, zeroed
+ * here) receives the histogram of the valid keys.  keys or counts may be NULL. */
+int lf2mi_bin_positions(const double *pos, int64_t np, int N, int32_t *keys, int32_t *counts,
+                        void *stream);
+
 /* synthetic initial conditions generated on the device (counter-based RNG keyed by
  * seed and particle index; positions produced in FP64 and binned as in upload):
  *   lattice: ng^3 particles on i/ng plus a Gaussian displacement of sigma_cells/N per

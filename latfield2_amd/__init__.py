@@ -25,7 +25,7 @@ from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64
                    PROJ_VECTOR, PROJ_VECTOR_NGP, PROJ_VECVEC, Lf2miError, call, load)
 
 __all__ = [
-    "Lattice", "Field", "KField", "CField", "PlanFFT", "Particles", "projection_init",
+    "Lattice", "Field", "KField", "CField", "PlanFFT", "Particles", "projection_init", "bin_positions",
     "scalarProjectionCIC_project", "scalarProjectionCIC_comm", "vectorProjectionCIC_project",
     "vectorProjectionCIC_comm", "VecVecProjectionCIC_project", "VecVecProjectionCIC_comm",
     "vectorProjectionCICNGP_project", "vectorProjectionCICNGP_comm", "symtensorProjectionCICNGP_project",
@@ -386,6 +386,21 @@ class Particles:
              float(dt_kick), float(dt_drift),
              ctypes.byref(vm) if want_vmax else None, _stream())
         return vm.value if want_vmax else None
+
+
+def bin_positions(pos, N: int, want_keys: bool = True, want_counts: bool = True):
+    """Particles::getPartCoord keys of FP64 positions (a (np, 3) device tensor) and the
+    per-cell histogram (particles/LATfield2_Particles.hpp:464-467); key -1 for a negative
+    coordinate.  Returns (keys, counts), either None when not asked for."""
+    torch = _torch()
+    pos = pos.contiguous()
+    if pos.dtype != torch.float64 or pos.dim() != 2 or pos.shape[1] != 3:
+        raise ValueError("bin_positions takes an (np, 3) float64 tensor")
+    keys = torch.empty(pos.shape[0], dtype=torch.int32, device=pos.device) if want_keys else None
+    counts = torch.empty(N ** 3, dtype=torch.int32, device=pos.device) if want_counts else None
+    call("lf2mi_bin_positions", _ptr(pos), pos.shape[0], int(N), None if keys is None else _ptr(keys),
+         None if counts is None else _ptr(counts), _stream())
+    return keys, counts
 
 
 def projection_init(f: Field):

@@ -68,6 +68,7 @@ SIGNATURES = {
     "lf2mi_parts_download_aos": [_vp, _vp, _i64, _i64, _i, _i64, _i64, _i64, _i64p, _vp],
     "lf2mi_parts_mass_from": [_vp, _i, _i64, _vp],
     "lf2mi_parts_vector_from": [_vp, _i64, _i],
+    "lf2mi_bin_positions": [_vp, _i64, _i, _vp, _vp, _vp],
     "lf2mi_parts_ic_lattice": [_vp, _i, _d, _d, _u64, _vp],
     "lf2mi_parts_ic_uniform": [_vp, _i64, _d, _u64, _vp],
     "lf2mi_parts_ic_blobs": [_vp, _i64, _i, _d, _d, _d, _d, _u64, _vp],

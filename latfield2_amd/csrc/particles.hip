@@ -1755,6 +1755,19 @@ __global__ void cell_count_kernel(int64_t np, const int32_t *__restrict__ cell, 
     if (i < np && cell[i] >= 0) atomicAdd(&counts[cell[i]], 1);
 }
 
+// keys of FP64 positions (AoS x,y,z) without a container; counts of the valid keys
+__global__ void bin_keys_kernel(int64_t np, const double *__restrict__ pos, int N, int32_t *__restrict__ keys,
+                                int32_t *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int cell;
+    float off[3];
+    bin_one(p, N, cell, off);
+    if (keys) keys[i] = cell;
+    if (counts && cell >= 0) atomicAdd(&counts[cell], 1);
+}
+
 __global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -2450,6 +2463,18 @@ int lf2mi_parts_download_box(lf2mi_parts p, const int *lo, const int *hi, int64_
     if (e != hipSuccess) return hip_fail(e, "lf2mi_parts_download_box", __FILE__, __LINE__);
     *n_out = cnt;
     if (cnt > max_out) return fail(LF2MI_ERANGE, "%d particles in the box exceed max_out %lld", cnt, (long long)max_out);
+    return LF2MI_OK;
+}
+
+int lf2mi_bin_positions(const double *pos, int64_t np, int N, int32_t *keys, int32_t *counts, void *stream) {
+    if (np < 0 || (np > 0 && !pos) || N < 1 || N > 1290) return fail(LF2MI_EINVAL, "lf2mi_bin_positions: bad argument");
+    hipStream_t st = as_stream(stream);
+    if (counts)
+        if (int rc = fill_int(counts, (int64_t)N * N * N, 0, st)) return rc;
+    if (np == 0 || (!keys && !counts)) return LF2MI_OK;
+    LF2MI_PROF("bin_keys", st);
+    bin_keys_kernel<<<grid1(np), 256, 0, st>>>(np, pos, N, keys, counts);
+    LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
 

@@ -73,6 +73,26 @@ def test_binning_bit_exact_with_edge_cases(prec):
         assert np.array_equal(d["pos"], pos[valid])
 
 
+def test_bin_positions_without_a_container_bit_exact():
+    """lf2mi_bin_positions: getPartCoord keys and the per-cell histogram of raw FP64
+    positions, the edge cases of the container test included"""
+    N = 32
+    announce(f"bin_positions N={N}")
+    pos, _, _ = _random_particles(30000, 4)
+    pos[0] = [0.0, 0.0, 0.0]
+    pos[1] = [1.0 - 2 ** -53, 0.5, 31 / 32]
+    pos[2] = [1.0, 0.25, 0.0]        # pos == box: key 0 along x
+    pos[3] = [-1e-9, 0.1, 0.1]       # negative: key -1, not counted
+    pos[4] = [5 / 32, 7 / 32 - 2 ** -60, 0.5]
+    pos[5] = [1.25, 0.5, 0.5]        # beyond the box: floor(pos N) % N wraps
+    keys, counts = lf.bin_positions(torch.as_tensor(pos, device="cuda"), N)
+    torch.cuda.synchronize()
+    assert np.array_equal(keys.cpu().numpy(), orc.bin_keys(pos, N))
+    assert np.array_equal(counts.cpu().numpy(), orc.cell_counts(pos, N))
+    k2, c2 = lf.bin_positions(torch.as_tensor(pos[:0], device="cuda"), N)  # empty: zero counts
+    assert k2.numel() == 0 and int(c2.abs().sum()) == 0
+
+
 def _oracle_deposits(pos, vel, mass, N, h, fold):
     rho = orc.real_field(N, h, 1)
     B = orc.real_field(N, h, 3)
