@@ -3,7 +3,7 @@
 plus a Gaussian displacement (sigma 1 cell), velocities Gaussian (sigma 0.01), mass 1/Np.
 Per kind: the scalar mass deposit, the 3-component vectorProjectionCIC (m v_i) and the
 6-component projection_Tij (m v_i v_j), each followed by R2C forward and C2R backward FFTs
-of every component.
+of every component; and the two CIC-NGP hybrids (deposit only).
 
 Parity: the FP64 oracle deposits the particles of a 256^3 window (cells [x0-1, x0+256] per
 axis, downloaded with lf2mi_parts_download_box) into a 512^3 periodic box shifted by
@@ -130,4 +130,34 @@ def test_large_projection_and_ffts(big, kind, C):
     assert e_rt <= 2e-6
     assert e_pars <= 1e-5
     del f, k, plan, kc
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kind,C", [(lf.PROJ_VECTOR_NGP, 3), (lf.PROJ_SYMTENSOR_NGP, 6)])
+def test_large_cic_ngp_projections(big, kind, C):
+    """vectorProjectionCICNGP / symtensorProjectionCICNGP at 1024^3 (projections.hpp:405-976):
+    the same window parity as the CIC kinds"""
+    parts, pos, vel, mass = big
+    name = {lf.PROJ_VECTOR_NGP: "vector CIC-NGP", lf.PROJ_SYMTENSOR_NGP: "symtensor CIC-NGP"}[kind]
+    announce(f"large projection {name}, C={C}")
+    lat = lf.Lattice(3, N, 1)
+    f = lf.Field(lat, C, lf.F32)
+    f.zero()
+    t_dep = _profile(lambda: parts.deposit(kind, f, lf.DEP_PERIODIC))
+    sl = tuple(slice(1 + x, 1 + x + W) for x in (X0[2], X0[1], X0[0]))  # [z, y, x]
+    got = f.t[sl].double().cpu().numpy()
+    orc.set_threads(16)
+    ref = orc.real_field(NP, 1, C)
+    if kind == lf.PROJ_VECTOR_NGP:
+        orc.vector_ngp_project(pos, vel, ref, NP, 1, mass=mass)
+    else:
+        orc.symtensor_ngp_project(pos, vel, ref, NP, 1, mass=mass)
+    ref = ref[3:3 + W, 3:3 + W, 3:3 + W] * (N / NP) ** 3
+    err = rel_l2(got, ref)
+    ms_dep = sum(t_dep.values())
+    dep_bytes = (28 + 4 * C) * N ** 3
+    print(f"{name}: deposit {ms_dep:.2f} ms ({dep_bytes / ms_dep / 1e6:.0f} GB/s) {t_dep}; "
+          f"window deposit rel-L2 {err:.2e}")
+    assert err <= 1e-6
+    del f
     torch.cuda.empty_cache()
