@@ -76,10 +76,12 @@ def sparse_particles(torch, parts, N: int) -> int:
     return int(c[(c > 0) & (c < GKD_DIRECT_MAX)].sum().item())
 
 
-def timed_steps(torch, lf, parts, plan, rho, phi, steps, warmup, dt):
+def timed_steps(torch, lf, parts, plan, rho, phi, steps, warmup, dt, on_timed_start=None):
     for _ in range(warmup):
         lf.pm_step(parts, plan, rho, phi, dt, dt)
     torch.cuda.synchronize()
+    if on_timed_start is not None:
+        on_timed_start()
     t0 = time.perf_counter()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record()
@@ -135,6 +137,8 @@ def main():
     ap.add_argument("--calib", action="store_true",
                     help="start with a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.steps < 1 or args.warmup < 0:
+        ap.error("--steps must be at least 1 and --warmup at least 0")
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
@@ -162,21 +166,22 @@ def main():
     log(f"FP32 workload: N={N}, Np={Np}, blobs IC")
     parts, rho, phi, plan = make_workload(lf, N, Np, lf.F32)
     torch.cuda.synchronize()
-    # timed region: K steps, per-kernel HIP events on the launch stream
-    _lib.profile_reset()
-    _lib.profile_enable(True)
-    ms, wall_ms = timed_steps(torch, lf, parts, plan, rho, phi, args.steps, args.warmup, dt)
+    # timed region: K steps, per-kernel HIP events on the launch stream (recorded from the
+    # first timed step on: the warmup steps are not in the registry)
+    def start_profile():
+        _lib.profile_reset()
+        _lib.profile_enable(True)
+
+    ms, wall_ms = timed_steps(torch, lf, parts, plan, rho, phi, args.steps, args.warmup, dt, start_profile)
     _lib.profile_enable(False)
     prof = _lib.profile_read()
     log(f"FP32 step {ms:.3f} ms (wall {wall_ms:.3f} ms)")
     np_sparse = sparse_particles(torch, parts, N)
-    # per-kernel launch averages, from the timed steps only (warmup launches included in
-    # the registry are removed by scaling with the launch counts of the timed region)
+    # per-kernel launch averages over the timed steps
     kernels = {}
     for name, (tot, cnt) in prof.items():
-        per_step = cnt / (args.steps + args.warmup)
-        kernels[name] = {"avg_ms": tot / cnt, "launches_per_step": per_step,
-                         "ms_per_step": tot / (args.steps + args.warmup)}
+        kernels[name] = {"avg_ms": tot / cnt, "launches_per_step": cnt / args.steps,
+                         "ms_per_step": tot / args.steps}
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     ab = algorithmic_bytes(dom, N, Np, 4, np_sparse)
     avg = kernels[dom]["avg_ms"]
