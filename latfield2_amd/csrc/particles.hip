@@ -706,9 +706,9 @@ __device__ __forceinline__ void window_add(unsigned long long *nd, int cell, int
 
 // phase B for one component: the lane's run of sorted particles (cells key[k], -1 past the
 // item's end), its last cell carried across the wave
-template <int KIND, int COMP, typename ACC, int BC>
-__device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int NWE, const int (&key)[DEP_PER],
-                                                  const float4 (&pd)[DEP_PER], const float4 (&pv)[DEP_PER],
+template <int KIND, int COMP, typename ACC, int BC, int PER = DEP_PER>
+__device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int NWE, const int (&key)[PER],
+                                                  const float4 (&pd)[PER], const float4 (&pv)[PER],
                                                   int ckey, bool any_shared, bool carry_end, unsigned long long *nodes,
                                                   double scale) {
     constexpr int MASK = Comp<KIND, COMP>::MASK, NGP = Comp<KIND, COMP>::NGP;
@@ -723,7 +723,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
     for (int k = 0; k < NS; ++k) acc[k] = ACC(0);
     int cur = -1;
 #pragma unroll
-    for (int p = 0; p < DEP_PER; ++p) {
+    for (int p = 0; p < PER; ++p) {
         const int c = key[p];
         if (c < 0) continue;  // past the end: every later particle of the run is too
         if (c != cur) {
@@ -1000,6 +1000,10 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
 // rather than once per item.  With one mass the fixed-point bound is n m, known before any
 // particle is read.  Phases as in deposit_f32_kernel; the window has its own LDS here.
 static constexpr int DEP_MERGE = 8;
+// consecutive items of a group sorted together (a lane's run holds DEP_SORT * DEP_PER sorted
+// particles): 1, since the larger LDS planes of 2 or 4 cost more occupancy than the longer
+// runs save
+static constexpr int DEP_SORT = 1;
 
 template <typename ACC, int BC>
 __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
@@ -1007,10 +1011,15 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
     constexpr int NW3 = NWMAX * NWMAX * NWMAX;
     constexpr int NCMAX = MAXB * MAXB * MAXB;
     constexpr bool FX = sizeof(ACC) == 8;
+    constexpr int SP = DEP_SORT * ITEM_P;  // particles per sort
+    constexpr int PER = SP / DEP_WG;       // sorted particles per lane run
+    static_assert(PER % 4 == 0, "runs are read as 16-byte rows");
+    static_assert(DEP_MERGE % DEP_SORT == 0, "whole sorts per group");
+    static_assert(2 * DEP_WG >= NCMAX, "two cells per thread in the scan");
     __shared__ int cst[NCMAX];
     __shared__ int scan_sh[DEP_WG / 64];
-    __shared__ __align__(16) float pl[3][ITEM_P];
-    __shared__ __align__(16) unsigned short skey[ITEM_P];
+    __shared__ __align__(16) float pl[3][SP];
+    __shared__ __align__(16) unsigned short skey[SP];
     __shared__ unsigned long long nodes[NW3];
     ItemRange r;
     if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
@@ -1023,12 +1032,12 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
     const DepMax m{__float_as_uint(A.mass_nv), {0u, 0u, 0u}};
     const double sc = FX ? 1.0 : window_scale<0, 0>(end - r.p0, m);
     for (int i = threadIdx.x; i < NW3; i += DEP_WG) nodes[i] = 0ull;
-    // the next item's particles are loaded before the current item's lane runs
-    int nc[DEP_PER];
-    float3 nq[DEP_PER];
+    // the next sort's particles are loaded before the current sort's lane runs
+    int nc[PER];
+    float3 nq[PER];
     auto fetch = [&](int q0) {
 #pragma unroll
-        for (int k = 0; k < DEP_PER; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int p = q0 + threadIdx.x + k * DEP_WG;
             nc[k] = -1;
             if (p < end) {
@@ -1038,16 +1047,16 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
         }
     };
     fetch(r.p0);
-    for (int p0 = r.p0; p0 < end; p0 += ITEM_P) {
-        const int n = min(ITEM_P, end - p0);
+    for (int p0 = r.p0; p0 < end; p0 += SP) {
+        const int n = min(SP, end - p0);
         for (int i = threadIdx.x; i < NC; i += DEP_WG) cst[i] = 0;
         __syncthreads();
         // A. rank by cell
         {
-            int lc[DEP_PER], rk[DEP_PER];
-            float3 qd[DEP_PER];
+            int lc[PER], rk[PER];
+            float3 qd[PER];
 #pragma unroll
-            for (int k = 0; k < DEP_PER; ++k) {
+            for (int k = 0; k < PER; ++k) {
                 lc[k] = -1;
                 qd[k] = nq[k];
                 if (nc[k] >= 0) {
@@ -1068,7 +1077,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < DEP_PER; ++k) {
+            for (int k = 0; k < PER; ++k) {
                 if (lc[k] >= 0) {
                     const int j = cst[lc[k]] + rk[k];
                     skey[j] = (unsigned short)lc[k];
@@ -1079,35 +1088,38 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
             }
         }
         __syncthreads();
-        // B. lane runs into the window
-        int key[DEP_PER];
-        float4 pd[DEP_PER], pv[DEP_PER];
-        const int j0 = DEP_PER * threadIdx.x;
-        {
-            const float4 ux = *reinterpret_cast<const float4 *>(&pl[0][j0]);
-            const float4 uy = *reinterpret_cast<const float4 *>(&pl[1][j0]);
-            const float4 uz = *reinterpret_cast<const float4 *>(&pl[2][j0]);
-            const ushort4 kk = *reinterpret_cast<const ushort4 *>(&skey[j0]);
-            const float q = A.mass_nv;
-            pd[0] = make_float4(ux.x, uy.x, uz.x, q);
-            pd[1] = make_float4(ux.y, uy.y, uz.y, q);
-            pd[2] = make_float4(ux.z, uy.z, uz.z, q);
-            pd[3] = make_float4(ux.w, uy.w, uz.w, q);
-            key[0] = j0 + 0 < n ? (int)kk.x : -1;
-            key[1] = j0 + 1 < n ? (int)kk.y : -1;
-            key[2] = j0 + 2 < n ? (int)kk.z : -1;
-            key[3] = j0 + 3 < n ? (int)kk.w : -1;
+        // B. lane runs into the window (16-byte rows of each plane; values past the end
+        // are masked by their key)
+        int key[PER];
+        float4 pd[PER], pv[PER];
+        const int j0 = PER * threadIdx.x;
+        const float q = A.mass_nv;
+#pragma unroll
+        for (int h = 0; h < PER / 4; ++h) {
+            const int jr = j0 + 4 * h;
+            const float4 ux = *reinterpret_cast<const float4 *>(&pl[0][jr]);
+            const float4 uy = *reinterpret_cast<const float4 *>(&pl[1][jr]);
+            const float4 uz = *reinterpret_cast<const float4 *>(&pl[2][jr]);
+            const ushort4 kk = *reinterpret_cast<const ushort4 *>(&skey[jr]);
+            pd[4 * h + 0] = make_float4(ux.x, uy.x, uz.x, q);
+            pd[4 * h + 1] = make_float4(ux.y, uy.y, uz.y, q);
+            pd[4 * h + 2] = make_float4(ux.z, uy.z, uz.z, q);
+            pd[4 * h + 3] = make_float4(ux.w, uy.w, uz.w, q);
+            key[4 * h + 0] = jr + 0 < n ? (int)kk.x : -1;
+            key[4 * h + 1] = jr + 1 < n ? (int)kk.y : -1;
+            key[4 * h + 2] = jr + 2 < n ? (int)kk.z : -1;
+            key[4 * h + 3] = jr + 3 < n ? (int)kk.w : -1;
         }
-        if (p0 + ITEM_P < end) fetch(p0 + ITEM_P);
+        if (p0 + SP < end) fetch(p0 + SP);
         int ckey = -1;
 #pragma unroll
-        for (int k = 0; k < DEP_PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
+        for (int k = 0; k < PER; ++k) ckey = key[k] >= 0 ? key[k] : ckey;
         const int next = __shfl_down(ckey, 1);
         const int prev = __shfl_up(ckey, 1);
         const bool any_shared = __ballot(lane > 0 && ckey >= 0 && prev == ckey) != 0ull;
         const bool carry_end = ckey >= 0 && (lane == 63 || next != ckey);
-        deposit_component<0, 0, ACC, BC>(A, B, NWE, key, pd, pv, ckey, any_shared, carry_end, nodes, sc);
-        __syncthreads();  // the planes and counts are reused by the next item
+        deposit_component<0, 0, ACC, BC, PER>(A, B, NWE, key, pd, pv, ckey, any_shared, carry_end, nodes, sc);
+        __syncthreads();  // the planes and counts are reused by the next sort
     }
     // C. the window into the field
     const int nn = NWE * NWE * NWE;
