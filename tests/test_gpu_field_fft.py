@@ -198,6 +198,30 @@ def test_c2c_full_size_round_trip_and_plane_wave_fp32(N):
     assert err.item() <= 2e-6, err.item()
 
 
+@pytest.mark.parametrize("c2c", [False, True])
+def test_fp64_round_trips_at_1024(c2c):
+    """R2C/C2R and C2C at 1024^3 in FP64 (the 128 KB strided tiles and the 144 KB C2C z tiles
+    of the FP64 path): backward(forward(f)) / N^3 = f to 1e-12 relative (max over the mesh)"""
+    N = 1024
+    announce(f"fp64 round trip 1024 c2c={c2c}")
+    lat = lf.Lattice(3, N, 1)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    if c2c:
+        r, k = lf.CField(lat, 1, lf.F64), lf.CField(lf.Lattice(3, N, 0), 1, lf.F64)
+    else:
+        r, k = lf.Field(lat, 1, lf.F64), lf.KField(lat, 0, 1, lf.F64)
+    r.t.copy_(torch.randn(r.t.shape, generator=g, device="cuda", dtype=torch.float64))
+    f0 = r.interior().clone()
+    plan = lf.PlanFFT(r, k)
+    plan.execute(lf.FFT_FORWARD)
+    plan.execute(lf.FFT_BACKWARD)
+    torch.cuda.synchronize()
+    err = (r.interior() / N ** 3 - f0).abs().max().item() / f0.abs().max().item()
+    assert err <= 1e-12, err
+    del plan, r, k, f0
+    torch.cuda.empty_cache()
+
+
 def test_wave_test_kats_on_device_fp64():
     """benchmarks_tests/wave_test.cpp and wave_test_back.cpp (BoxSize 64, tolerance 1e-10)"""
     N, tol = 64, 1e-10
