@@ -501,7 +501,8 @@ class Particles {
         sync();
         if (oset == nullptr) {
             if (has_mass_member<part>::value)
-                lf2mi_or_die(lf2mi_parts_mass_from(h_, LF2MI_MASS_PART, mass_offset<part>(), nullptr), "projection mass");
+                lf2mi_or_die(lf2mi_parts_mass_from(h_, mass_source<part>(), mass_offset<part>(), nullptr),
+                             "projection mass");
             else
                 use_global_mass(info_.mass);
         } else if (flag_where == FROM_INFO) {
@@ -536,7 +537,9 @@ class Particles {
         lf2mi_or_die(lf2mi_parts_set_payload(h_, (int)sizeof(part)), "Particles::initialize");
         lf2mi_or_die(lf2mi_parts_upload_aos(h_, (int64_t)staged_.size(), staged_.data(), (int64_t)sizeof(part),
                                             lf2mi_real_code(), (int64_t)offsetof(part, pos), (int64_t)offsetof(part, vel),
-                                            (int64_t)offsetof(part, ID), pm ? mass_offset<part>() : -1, nullptr, nullptr),
+                                            (int64_t)offsetof(part, ID), pm && mass_source<part>() == LF2MI_MASS_PART
+                                                                             ? mass_offset<part>() : -1,
+                                            nullptr, nullptr),
                      "Particles::addParticle_global");
         staged_.clear();
         dirty_ = false;
@@ -544,10 +547,20 @@ class Particles {
     template <typename P>
     static int64_t mass_offset() {
         if constexpr (has_mass_member<P>::value) {
-            static_assert(sizeof(((P *)nullptr)->mass) == sizeof(double), "a per-particle mass is a double");
+            static_assert(sizeof(((P *)nullptr)->mass) == sizeof(double) || sizeof(((P *)nullptr)->mass) == sizeof(float),
+                          "a per-particle mass is a double or a float");
             return (int64_t)offsetof(P, mass);
         } else {
             return -1;
+        }
+    }
+    // a mass member is read with its declared type (a float one: Real mass under SINGLE)
+    template <typename P>
+    static int mass_source() {
+        if constexpr (has_mass_member<P>::value) {
+            return sizeof(((P *)nullptr)->mass) == sizeof(float) ? LF2MI_MASS_PART_F32 : LF2MI_MASS_PART;
+        } else {
+            return LF2MI_MASS_INFO;
         }
     }
     void use_global_mass(double m) {

@@ -215,7 +215,9 @@ int lf2mi_parts_set_payload(lf2mi_parts p, int bytes);
  * vel[3] of type Real (real_dtype LF2MI_F32 or LF2MI_F64) at byte offsets off_pos and
  * off_vel, the ID (int64) at off_id and a per-particle mass (double) at off_mass, each
  * -1 when absent (IDs then count from 0; masses need a container with masses or a
- * payload).  With a payload, stride must equal it and the structs are kept whole.
+ * payload).  With a payload, stride must equal it and the structs are kept whole; a
+ * container created with LF2MI_PARTS_MASS then accepts off_mass -1 and starts at mass 0
+ * until lf2mi_parts_mass_from gathers the masses (a float member, LF2MI_MASS_PART_F32).
  * Same filtering and replacement as lf2mi_parts_upload.  Syncs. */
 int lf2mi_parts_upload_aos(lf2mi_parts p, int64_t n, const void *base, int64_t stride, int real_dtype,
                            int64_t off_pos, int64_t off_vel, int64_t off_id, int64_t off_mass,
@@ -234,12 +236,16 @@ int lf2mi_parts_download_aos(lf2mi_parts p, void *base, int64_t n_room, int64_t 
  *   LF2MI_MASS_PART  per particle: the double at byte `offset` of its payload
  *                    (FROM_PART), gathered on the device into the per-particle masses;
  *                    offset -1 uses those masses as they stand (uploaded, or gathered
- *                    by an earlier call).
+ *                    by an earlier call);
+ *   LF2MI_MASS_PART_F32  the same from a float at `offset`: a mass member declared Real
+ *                    under SINGLE, read with its own type (the reference reads every
+ *                    per-particle mass as a double, projections.hpp:150).
  * The choice holds for later deposits and PM steps (whose mean density follows it).
  * A container created with LF2MI_PARTS_MASS starts on LF2MI_MASS_PART, others on
  * LF2MI_MASS_INFO.  Syncs. */
 #define LF2MI_MASS_INFO 0
 #define LF2MI_MASS_PART 1
+#define LF2MI_MASS_PART_F32 2 /* a float mass member (Real mass under SINGLE) */
 int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream);
 
 /* The vector a projection's oset[1] names (the vector and tensor projections read v at that

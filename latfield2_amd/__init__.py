@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART,
+from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART, MASS_PART_F32,
                    FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_SYMTENSOR_NGP,
                    PROJ_VECTOR, PROJ_VECTOR_NGP, PROJ_VECVEC, Lf2miError, call, load)
 
@@ -317,6 +317,7 @@ class Particles:
     def set_payload(self, nbytes: int):
         """keep each particle's whole struct (nbytes, a multiple of 8) on the device"""
         call("lf2mi_parts_set_payload", self.h, int(nbytes))
+        self._payload = int(nbytes)
 
     @staticmethod
     def _fields(dtype):
@@ -329,12 +330,21 @@ class Particles:
 
     def upload_structs(self, parts) -> int:
         """addParticle_global from a numpy structured array with fields pos (3 Real), vel
-        (3 Real), optional ID (int64) and mass (float64); returns the particles kept"""
+        (3 Real), optional ID (int64) and mass (float64, or float32 read with its own type);
+        returns the particles kept"""
         a = np.ascontiguousarray(parts)
         code, op, ov, oi, om = self._fields(a.dtype)
+        f4 = om >= 0 and a.dtype.fields["mass"][0] == np.float32
         kept = ctypes.c_int64(0)
         call("lf2mi_parts_upload_aos", self.h, a.shape[0], a.ctypes.data_as(ctypes.c_void_p), a.dtype.itemsize,
-             code, op, ov, oi, om, ctypes.byref(kept), _stream())
+             code, op, ov, oi, -1 if f4 else om, ctypes.byref(kept), _stream())
+        if f4:
+            # gathered from the kept structs on the device; the mass source stays as it was
+            if not getattr(self, "_payload", 0):
+                raise ValueError("a float32 mass field needs the structs kept on the device (set_payload)")
+            self.mass_from(MASS_PART_F32, om)
+            if not self.flags & PARTS_MASS:
+                self.mass_from(MASS_INFO)
         return kept.value
 
     def download_structs(self, dtype):
@@ -349,9 +359,9 @@ class Particles:
         return out[:n.value]
 
     def mass_from(self, where: int, offset: int = -1):
-        """mass of later deposits: MASS_INFO (global) or MASS_PART (the double at byte offset
-        of each kept struct, gathered into the per-particle masses; -1: those masses as they
-        stand)"""
+        """mass of later deposits: MASS_INFO (global), MASS_PART (the double at byte offset of
+        each kept struct, gathered into the per-particle masses; -1: those masses as they
+        stand) or MASS_PART_F32 (the float at byte offset)"""
         call("lf2mi_parts_mass_from", self.h, int(where), int(offset), _stream())
 
     def vector_from(self, offset: int = -1, real: int = F32):

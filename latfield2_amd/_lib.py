@@ -19,7 +19,7 @@ PROJ_SCALAR, PROJ_VECTOR, PROJ_VECVEC, PROJ_VECTOR_NGP, PROJ_SYMTENSOR_NGP = 0, 
 DEP_UNFOLDED, DEP_PERIODIC = 0, 1
 ACC_NATIVE, ACC_FIXED64 = 0, 1
 PARTS_ID, PARTS_MASS = 1, 2
-MASS_INFO, MASS_PART = 0, 1
+MASS_INFO, MASS_PART, MASS_PART_F32 = 0, 1, 2
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

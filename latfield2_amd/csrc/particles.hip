@@ -1933,16 +1933,17 @@ __global__ void sum_mass_kernel(int64_t np, const T *__restrict__ mass, double *
     if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
 }
 
-// per-particle masses from the double at byte `off` of each payload; sum into *total
-template <typename T>
+// per-particle masses from the value of type S (double, or float) at byte `off` of each
+// payload; sum into *total
+template <typename T, typename S>
 __global__ void payload_mass_kernel(int64_t np, const unsigned long long *__restrict__ pl, int plw, int64_t off,
                                     T *__restrict__ mass, double *__restrict__ total) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (i < np) {
         const unsigned char *b = reinterpret_cast<const unsigned char *>(pl + i * plw);
-        double v;
-        memcpy(&v, b + off, sizeof(double));
+        S v;
+        memcpy(&v, b + off, sizeof(S));
         mass[i] = (T)v;
         m = (double)(T)v;
     }
@@ -2306,7 +2307,11 @@ int lf2mi_parts_upload_aos(lf2mi_parts p, int64_t n, const void *base, int64_t s
         if (off_mass >= 0) memcpy(&mass[(size_t)i], q + off_mass, 8);
     }
     if (p->plw) p->aos_vel = off_vel;
-    return upload_impl(p, n, pos.data(), vel.data(), off_mass >= 0 ? mass.data() : nullptr,
+    // a container with per-particle masses whose structs carry none readable as a double (a
+    // float member, gathered afterwards by lf2mi_parts_mass_from) starts at mass 0
+    const bool zero_mass = off_mass < 0 && (p->flags & LF2MI_PARTS_MASS) && p->plw;
+    if (zero_mass) mass.assign((size_t)n, 0.0);
+    return upload_impl(p, n, pos.data(), vel.data(), (off_mass >= 0 || zero_mass) ? mass.data() : nullptr,
                        off_id >= 0 ? id.data() : nullptr, p->plw ? b : nullptr, kept, stream);
 }
 
@@ -2363,7 +2368,8 @@ int lf2mi_parts_download_aos(lf2mi_parts p, void *base, int64_t n_room, int64_t 
 
 int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream) {
     if (!p) return fail(LF2MI_EINVAL, "null container");
-    if (where != LF2MI_MASS_INFO && where != LF2MI_MASS_PART) return fail(LF2MI_EINVAL, "bad mass source %d", where);
+    if (where != LF2MI_MASS_INFO && where != LF2MI_MASS_PART && where != LF2MI_MASS_PART_F32)
+        return fail(LF2MI_EINVAL, "bad mass source %d", where);
     if (where == LF2MI_MASS_INFO) {
         p->mass_src = 0;
         p->total_mass = p->mass_global * (double)p->np;
@@ -2371,7 +2377,8 @@ int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream
     }
     const bool has_mass = p->dtype == LF2MI_F32 ? p->b32[0].mass != nullptr : p->b64.mass != nullptr;
     if (!has_mass) return fail(LF2MI_EINVAL, "per-particle masses need a container with masses or a payload");
-    if (offset >= 0 && (!p->plw || offset + 8 > (int64_t)p->plw * 8)) {
+    const int64_t msize = where == LF2MI_MASS_PART_F32 ? 4 : 8;
+    if (offset >= 0 && (!p->plw || offset + msize > (int64_t)p->plw * 8)) {
         return fail(LF2MI_EINVAL, "mass offset %lld outside the payload", (long long)offset);
     }
     hipStream_t st = as_stream(stream);
@@ -2381,12 +2388,17 @@ int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream
     int rc = LF2MI_OK;
     if (p->np > 0) {
         if (offset >= 0) {
-            if (p->dtype == LF2MI_F32)
-                payload_mass_kernel<float><<<grid1(p->np), 256, 0, st>>>(p->np, p->b32[p->cur].pl, p->plw, offset,
-                                                                          p->b32[p->cur].mass, dsum);
-            else
-                payload_mass_kernel<double><<<grid1(p->np), 256, 0, st>>>(p->np, p->b64.pl, p->plw, offset, p->b64.mass,
-                                                                           dsum);
+            const bool f4 = where == LF2MI_MASS_PART_F32;
+            const unsigned g = grid1(p->np);
+            if (p->dtype == LF2MI_F32) {
+                Soa32 &s = p->b32[p->cur];
+                if (f4) payload_mass_kernel<float, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
+                else payload_mass_kernel<float, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
+            } else {
+                Soa64 &s = p->b64;
+                if (f4) payload_mass_kernel<double, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
+                else payload_mass_kernel<double, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
+            }
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = hip_fail(e, "payload_mass_kernel", __FILE__, __LINE__);
         } else if (p->dtype == LF2MI_F32) {

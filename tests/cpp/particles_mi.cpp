@@ -30,6 +30,14 @@ struct part_weighted {
     Real mom[3];
 };
 
+// a part type whose mass is a Real (a float under SINGLE), read with its declared type
+struct part_real_mass {
+    long ID;
+    Real pos[3];
+    Real vel[3];
+    Real mass;
+};
+
 // sum of the interior of a scalar field (host copy)
 static double interior_sum(Field<Real> &f, Lattice &lat) {
     f.toHost();
@@ -208,6 +216,32 @@ int main(int argc, char **argv) {
         }
         std::printf("struct round trip after a move: %ld mismatches\n", bad);
         if (bad) ++fails;
+    }
+    // a Real mass member (projections.hpp:150 reads it as a double; here with its own type)
+    {
+        Particles<part_real_mass, part_simple_info, part_simple_dataType> rp;
+        rp.initialize(info, dtype, &lat_part, box);
+        double msum = 0.0;
+        long k = 0;
+        for (xp.first(); xp.test(); xp.next()) {
+            part_real_mass p;
+            p.ID = k;
+            for (int d = 0; d < 3; ++d) {
+                p.pos[d] = ((Real)xp.coord(d) + (Real)0.25) / N;
+                p.vel[d] = 0.0;
+            }
+            p.mass = (Real)(0.5 + 0.125 * (k % 3));
+            msum += p.mass;
+            rp.addParticle_global(p);
+            ++k;
+        }
+        projection_init(&rho);
+        scalarProjectionCIC_project(&rp, &rho);
+        scalarProjectionCIC_comm(&rho);
+        const double s_real = interior_sum(rho, lat) / ((double)N * N * N);
+        const double stol = sizeof(Real) == 8 ? 1e-10 : 2e-4;
+        std::printf("Real mass member (%zu bytes): deposit %.9g (%.9g)\n", sizeof(Real), s_real, msum);
+        if (std::fabs(s_real / msum - 1) > stol) ++fails;
     }
     std::printf("particles_mi N=%d %s: %d failed checks\n", N, sizeof(Real) == 8 ? "double" : "single", fails);
     return fails;

@@ -131,6 +131,38 @@ def test_structs_follow_their_particles_through_moves(prec):
         assert rel_l2(got, _oracle(np.ascontiguousarray(d["pos"][order]), ref["w"], N)) <= TOL[prec], step
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_float_mass_member_read_with_its_own_type(prec):
+    """a `Real mass` member under SINGLE is a float: read as a float (the reference reads every
+    per-particle mass as a double, projections.hpp:150)"""
+    N, n = 16, 4000
+    announce(f"structs float mass {prec}")
+    dt = np.dtype([("ID", "<i8"), ("pos", "<f4", 3), ("vel", "<f4", 3), ("mass", "<f4"), ("pad", "<f4")], align=True)
+    rng = np.random.default_rng(21)
+    a = np.zeros(n, dtype=dt)
+    a["ID"] = np.arange(n)
+    a["pos"] = rng.random((n, 3))
+    a["mass"] = rng.random(n) + 0.25
+    parts = lf.Particles(N, n, DT[prec], ids=True, per_particle_mass=True, mass=0.7)
+    parts.set_payload(dt.itemsize)
+    assert parts.upload_structs(a) == n
+    d = parts.download()
+    order = np.argsort(d["id"])
+    pos = np.ascontiguousarray(d["pos"][order])
+    got = _deposit(parts, N, prec)
+    assert rel_l2(got, _oracle(pos, a["mass"].astype(np.float64), N)) <= TOL[prec]
+    # an explicit FROM_PART of the float member, then back to the global mass
+    parts.mass_from(lf.MASS_INFO)
+    parts.mass_from(lf.MASS_PART_F32, dt.fields["mass"][1])
+    got = _deposit(parts, N, prec)
+    assert rel_l2(got, _oracle(pos, a["mass"].astype(np.float64), N)) <= TOL[prec]
+    parts.mass_from(lf.MASS_INFO)
+    got = _deposit(parts, N, prec)
+    assert rel_l2(got, _oracle(pos, np.full(n, 0.7), N)) <= TOL[prec]
+    with pytest.raises(lf.Lf2miError):
+        parts.mass_from(lf.MASS_PART_F32, dt.itemsize - 2)  # past the struct
+
+
 _VEC_KINDS = [(lf.PROJ_VECTOR, 3, "vector_project"), (lf.PROJ_VECVEC, 6, "vecvec_project"),
               (lf.PROJ_VECTOR_NGP, 3, "vector_ngp_project"), (lf.PROJ_SYMTENSOR_NGP, 6, "symtensor_ngp_project")]
 
