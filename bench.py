@@ -83,14 +83,17 @@ def timed_steps(torch, lf, parts, plan, rho, phi, steps, warmup, dt, on_timed_st
     if on_timed_start is not None:
         on_timed_start()
     t0 = time.perf_counter()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(steps):
+    # one event per step boundary: the mean over the K steps, and each step's own time
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record()
+    for i in range(steps):
         lf.pm_step(parts, plan, rho, phi, dt, dt)
-    ev1.record()
+        ev[i + 1].record()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3
-    return ev0.elapsed_time(ev1) / steps, wall / steps
+    per_step = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    median = per_step[steps // 2] if steps % 2 else 0.5 * (per_step[steps // 2 - 1] + per_step[steps // 2])
+    return ev[0].elapsed_time(ev[steps]) / steps, wall / steps, median
 
 
 def cpu_baseline(lf, torch, threads):
@@ -172,10 +175,10 @@ def main():
         _lib.profile_reset()
         _lib.profile_enable(True)
 
-    ms, wall_ms = timed_steps(torch, lf, parts, plan, rho, phi, args.steps, args.warmup, dt, start_profile)
+    ms, wall_ms, med_ms = timed_steps(torch, lf, parts, plan, rho, phi, args.steps, args.warmup, dt, start_profile)
     _lib.profile_enable(False)
     prof = _lib.profile_read()
-    log(f"FP32 step {ms:.3f} ms (wall {wall_ms:.3f} ms)")
+    log(f"FP32 step {ms:.3f} ms (median {med_ms:.3f} ms, wall {wall_ms:.3f} ms)")
     np_sparse = sparse_particles(torch, parts, N)
     # per-kernel launch averages over the timed steps
     kernels = {}
@@ -238,7 +241,7 @@ def main():
         extra["fp32_vs_fp64_phi_rel_l2"] = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
         del p32, r32, f32_, pl32
         torch.cuda.empty_cache()
-        ms64, _ = timed_steps(torch, lf, p64, pl64, r64, f64, max(2, args.steps // 3), 1, dt)
+        ms64, _, _ = timed_steps(torch, lf, p64, pl64, r64, f64, max(2, args.steps // 3), 1, dt)
         extra["fp64_ms_per_step"] = round(ms64, 3)
         extra["fp64_over_fp32"] = round(ms64 / ms, 2)
         log(f"FP64 step {ms64:.3f} ms, phi rel-L2 FP32 vs FP64 {extra['fp32_vs_fp64_phi_rel_l2']:.2e}")
@@ -263,6 +266,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 3),
+        "ms_per_step_median": round(med_ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
