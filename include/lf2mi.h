@@ -58,7 +58,7 @@ extern "C" {
 
 /* deposit accumulation */
 #define LF2MI_ACC_NATIVE 0  /* atomics in the container's precision */
-#define LF2MI_ACC_FIXED64 1 /* int64 fixed point, scale 2^-32: run-to-run reproducible */
+#define LF2MI_ACC_FIXED64 1 /* int64 fixed point: run-to-run reproducible (see lf2mi_deposit) */
 
 /* particle container flags */
 #define LF2MI_PARTS_ID 1   /* keep 64-bit particle IDs (part_simple::ID) */
@@ -297,7 +297,11 @@ int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha,
  *   target LF2MI_DEP_UNFOLDED adds exactly what the reference _project adds (ghost
  *          nodes included, follow with lf2mi_projection_comm); LF2MI_DEP_PERIODIC adds
  *          the folded result into the interior (== _project + _comm on the interior).
- *   acc   LF2MI_ACC_NATIVE or LF2MI_ACC_FIXED64 (F32 scalar only).
+ *   acc   LF2MI_ACC_NATIVE, or LF2MI_ACC_FIXED64 (F32 containers): every particle's term is
+ *         rounded to an int64 fixed point and summed in integers, so the result does not
+ *         depend on the order of the additions (bitwise reproducible); scalar kind: 2^-32 of
+ *         the mass per unit weight; vector and tensor kinds: per component 2^(e-62) with
+ *         np max|q| max|v_a| max|v_b| < 2^e (finding the maxima synchronises the stream).
  * Values are added (+=): zero the field first, as projection_init does. */
 int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc,
                   void *stream);

@@ -613,7 +613,8 @@ struct DepArgs {
     float mass_nv;           // global mass * N^3 (mass / res^3)
     float *field;            // native accumulation target
     unsigned long long *fx;  // fixed-point target (KIND 0 only)
-    double fxscale;          // 2^32 / mass_nv: fixed point counts dimensionless weights
+    double fxs[6];           // fixed-point scale per component (scalar: 2^32 / mass_nv, so the
+                             // fixed point counts dimensionless weights)
     int h;
     int tgt;
 };
@@ -757,7 +758,7 @@ __device__ __forceinline__ void deposit_component(const DepArgs &A, int B, int N
             for (int ky = 0; ky < SY; ++ky)
 #pragma unroll
                 for (int kx = 0; kx < SX; ++kx)
-                    acc[kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxscale);
+                    acc[kx + SX * (ky + SY * kz)] += AccOps<ACC>::term(q, wx[kx] * wy[ky] * wz[kz], A.fxs[COMP]);
     }
     // the run's last cell: summed over the lanes that share it (uniform branch)
     if (any_shared) seg_scan(ckey, acc);
@@ -1138,6 +1139,45 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
         if constexpr (FX) AccOps<ACC>::flush(A, idx, w);
         else atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
     }
+}
+
+// maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
+// non-negative floats: the bound of the vector and tensor fixed-point deposits
+__global__ void dep_max_kernel(int64_t np, Soa32 s, float mass_nv, unsigned int *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float m[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i < np) {
+        m[0] = fabsf(s.mass ? s.mass[i] * mass_nv : mass_nv);
+        m[1] = fabsf(s.vel[0][i]);
+        m[2] = fabsf(s.vel[1][i]);
+        m[3] = fabsf(s.vel[2][i]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[k] = fmaxf(m[k], __shfl_xor(m[k], o));
+    }
+    if (lane_id() == 0)
+        for (int k = 0; k < 4; ++k) atomicMax(&out[k], __float_as_uint(m[k]));
+}
+
+struct FxScales {
+    double inv[6];
+};
+
+// component c of site i at fx[i C + c]: f += q / scale_c
+__global__ void fixed_to_field_c_kernel(const unsigned long long *__restrict__ fx, float *__restrict__ f, int64_t n,
+                                        int C, FxScales S) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const long long q = (long long)fx[i];
+        if (q) f[i] += (float)((double)q * S.inv[i % C]);
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long *__restrict__ p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
 __global__ void fixed_to_field_kernel(const unsigned long long *__restrict__ fx, float *__restrict__ f, int64_t n,
@@ -2647,9 +2687,10 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     A.tgt = target;
     const unsigned g = (unsigned)item_grid(p);
     if (acc == LF2MI_ACC_FIXED64) {
-        if (kind != LF2MI_PROJ_SCALAR) return fail(LF2MI_EUNSUPPORTED, "fixed-point deposit supports the scalar projection");
+        static const int comps[5] = {1, 3, 6, 3, 6};
+        const int C = comps[kind];
         const int64_t G = N + 2 * halo;
-        const int64_t n = G * G * G;
+        const int64_t n = G * G * G * C;
         if (p->fx_n < n) {
             if (p->fx) hipFree(p->fx);
             p->fx = nullptr;
@@ -2657,20 +2698,69 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
             LF2MI_HIP(hipMalloc((void **)&p->fx, n * sizeof(unsigned long long)));
             p->fx_n = n;
         }
-        LF2MI_HIP(hipMemsetAsync(p->fx, 0, n * sizeof(unsigned long long), st));
-        // accumulate dimensionless weights (times the per-particle mass ratio) in 2^-32 units
-        const double mref = A.mass_nv;
+        // cleared by a kernel (a memset node would misbehave in a captured graph)
+        fill_u64_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(p->fx, n);
+        LF2MI_LAUNCHED();
         A.fx = p->fx;
-        A.fxscale = 4294967296.0 / mref;
+        FxScales S{};
+        if (kind == LF2MI_PROJ_SCALAR) {
+            // dimensionless weights (times the per-particle mass ratio) in 2^-32 units
+            const double mref = A.mass_nv;
+            A.fxs[0] = 4294967296.0 / mref;
+            S.inv[0] = mref / 4294967296.0;
+        } else {
+            // per component a power-of-two scale 2^(62 - e) with np max|q| max|v_a| max|v_b| < 2^e:
+            // no node (nor any partial sum) can leave the int64 range, the quantum is 2^-62 of
+            // that bound, and the integer sums do not depend on the order of the particles
+            unsigned int *dmax = nullptr, hmax[4] = {0u, 0u, 0u, 0u};
+            LF2MI_HIP(hipMallocAsync((void **)&dmax, 4 * sizeof(unsigned int), st));
+            fill_int_kernel<<<1, 64, 0, st>>>(reinterpret_cast<int *>(dmax), 4, 0);
+            dep_max_kernel<<<grid1(p->np), 256, 0, st>>>(p->np, A.s, A.mass_nv, dmax);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(hmax, dmax, sizeof hmax, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            hipFreeAsync(dmax, st);
+            if (e != hipSuccess) return hip_fail(e, "fixed-point bound", __FILE__, __LINE__);
+            float mx[4];
+            memcpy(mx, hmax, sizeof mx);
+            static const int va[5][6] = {{-1}, {0, 1, 2}, {0, 0, 0, 1, 1, 2}, {0, 1, 2}, {0, 0, 0, 1, 1, 2}};
+            static const int vb[5][6] = {{-1}, {-1, -1, -1}, {0, 1, 2, 1, 2, 2}, {-1, -1, -1}, {0, 1, 2, 1, 2, 2}};
+            for (int c = 0; c < C; ++c) {
+                double bound = (double)p->np * (double)mx[0] * (double)mx[1 + va[kind][c]];
+                if (vb[kind][c] >= 0) bound *= (double)mx[1 + vb[kind][c]];
+                double sc = 1.0;
+                if (bound > 0.0 && bound < 1e300) {
+                    int ex;
+                    frexp(bound, &ex);
+                    sc = ldexp(1.0, 62 - ex);
+                }
+                A.fxs[c] = sc;
+                S.inv[c] = 1.0 / sc;
+            }
+        }
         {
-            LF2MI_PROF("deposit_f32_fixed64", st);
-            launch_deposit<0, unsigned long long>(A, p->g.B == 8, g, st);
+            static const char *const namesfx[5] = {"deposit_f32_fixed64", "deposit_f32_vector_fixed64",
+                                                   "deposit_f32_vecvec_fixed64", "deposit_f32_vector_ngp_fixed64",
+                                                   "deposit_f32_symtensor_ngp_fixed64"};
+            LF2MI_PROF(namesfx[kind], st);
+            const bool b8 = p->g.B == 8;
+            switch (kind) {
+                case 0: launch_deposit<0, unsigned long long>(A, b8, g, st); break;
+                case 1: launch_deposit<1, unsigned long long>(A, b8, g, st); break;
+                case 2: launch_deposit<2, unsigned long long>(A, b8, g, st); break;
+                case 3: launch_deposit<3, unsigned long long>(A, b8, g, st); break;
+                default: launch_deposit<4, unsigned long long>(A, b8, g, st); break;
+            }
             LF2MI_LAUNCHED();
         }
         {
             LF2MI_PROF("fixed_to_field", st);
-            fixed_to_field_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(
-                p->fx, (float *)field, n, mref / 4294967296.0);
+            if (C == 1)
+                fixed_to_field_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(
+                    p->fx, (float *)field, n, S.inv[0]);
+            else
+                fixed_to_field_c_kernel<<<(unsigned)std::min<int64_t>(grid1(n), 65536), 256, 0, st>>>(
+                    p->fx, (float *)field, n, C, S);
             LF2MI_LAUNCHED();
         }
         return LF2MI_OK;
