@@ -321,7 +321,10 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
 /* One PM step (the driver sequence of examples/poissonSolver.cpp plus the particle
  * update): zero rho, periodic CIC deposit, mean-subtracted Poisson solve with
  * G = -1/K^2 and 1/N^3, gather + kick + drift, re-bucket.  rho and phi are real fields
- * with ghost width `halo` in the container's precision; plan must match (N, C=1). */
+ * with ghost width `halo` in the container's precision; plan must match (N, C=1).
+ * After a first call (which sizes the buffers) a step only launches kernels on `stream`, so
+ * steps can be captured into a HIP graph; the container alternates between two particle
+ * buffers, so a capture that is replayed holds an even number of steps. */
 int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int halo,
                   double dt_kick, double dt_drift, int acc, void *stream);
 
