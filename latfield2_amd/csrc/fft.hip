@@ -70,24 +70,64 @@ template <int S, typename V> __device__ __forceinline__ void dft8(V *a) {
     a[7] = csub(e3, o3);
 }
 
+// a1' = a0 + w a1 + w^2 a2, a2' = a0 + w^2 a1 + w a2 with w = exp(S 2 pi i / 3)
+template <int S, typename V> __device__ __forceinline__ void dft3(V &a0, V &a1, V &a2) {
+    using T = decltype(a0.x);
+    const T h3 = (T)0.86602540378443864676;  // sqrt(3) / 2
+    const V t1 = cadd(a1, a2);
+    V t2;
+    t2.x = a0.x - (T)0.5 * t1.x;
+    t2.y = a0.y - (T)0.5 * t1.y;
+    V t3 = mul_si<S>(csub(a1, a2));
+    t3.x *= h3;
+    t3.y *= h3;
+    a0 = cadd(a0, t1);
+    a1 = cadd(t2, t3);
+    a2 = csub(t2, t3);
+}
+
 template <int R, int S, typename V> __device__ __forceinline__ void dft(V *a) {
     if constexpr (R == 2) dft2<S>(a[0], a[1]);
+    else if constexpr (R == 3) dft3<S>(a[0], a[1], a[2]);
     else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
     else dft8<S>(a);
 }
 
 // ---------------------------------------------------------------- Stockham passes
-// Tile of L lines of length N.  Radix plan: 8 = [8], 16 = [4,4], 32 = [8,4], 64 = [8,8],
-// 128 = [8,4,4], 256 = [8,8,4], 512 = [8,8,8], 1024 = [8,8,4,4].
-constexpr int fft_np(int N) { return N == 8 ? 1 : (N <= 64 ? 2 : (N <= 512 ? 3 : 4)); }
+// Tile of L lines of length N.  Radix plans (passes in order):
+//   powers of two: 8 = [8], 16 = [4,4], 32 = [8,4], 64 = [8,8], 128 = [8,4,4], 256 = [8,8,4],
+//                  512 = [8,8,8], 1024 = [8,8,4,4];
+//   3 * 2^k:       12 = [4,3], 24 = [8,3], 48 = [4,4,3], 96 = [8,4,3], 192 = [8,8,3],
+//                  384 = [8,4,4,3], 768 = [8,8,4,3].
+constexpr int fft_np(int N) {
+    return N == 8 ? 1
+         : (N == 16 || N == 32 || N == 64 || N == 12 || N == 24) ? 2
+         : (N == 128 || N == 256 || N == 512 || N == 48 || N == 96 || N == 192) ? 3
+         : 4;
+}
 constexpr int fft_radix(int N, int p) {
     return N == 16 ? 4
          : N == 32 ? (p == 0 ? 8 : 4)
          : N == 128 ? (p == 0 ? 8 : 4)
          : N == 256 ? (p < 2 ? 8 : 4)
          : N == 1024 ? (p < 2 ? 8 : 4)
+         : N == 12 ? (p == 0 ? 4 : 3)
+         : N == 24 ? (p == 0 ? 8 : 3)
+         : N == 48 ? (p < 2 ? 4 : 3)
+         : N == 96 ? (p == 0 ? 8 : (p == 1 ? 4 : 3))
+         : N == 192 ? (p < 2 ? 8 : 3)
+         : N == 384 ? (p == 0 ? 8 : (p < 3 ? 4 : 3))
+         : N == 768 ? (p < 2 ? 8 : (p == 2 ? 4 : 3))
          : 8;
 }
+// the product of the plan's radices is N
+constexpr int fft_prod(int N, int p) { return p == fft_np(N) ? 1 : fft_radix(N, p) * fft_prod(N, p + 1); }
+static_assert(fft_prod(8, 0) == 8 && fft_prod(16, 0) == 16 && fft_prod(32, 0) == 32 && fft_prod(64, 0) == 64 &&
+                  fft_prod(128, 0) == 128 && fft_prod(256, 0) == 256 && fft_prod(512, 0) == 512 &&
+                  fft_prod(1024, 0) == 1024 && fft_prod(12, 0) == 12 && fft_prod(24, 0) == 24 &&
+                  fft_prod(48, 0) == 48 && fft_prod(96, 0) == 96 && fft_prod(192, 0) == 192 &&
+                  fft_prod(384, 0) == 384 && fft_prod(768, 0) == 768,
+              "radix plans");
 constexpr int fft_ns(int N, int p) { return p == 0 ? 1 : fft_ns(N, p - 1) * fft_radix(N, p - 1); }
 
 // One radix-R pass after sub-transforms of size NS (Stockham auto-sort): butterfly j of
@@ -286,7 +326,7 @@ __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
         const int kx = q / C, c = q % C;
         const int l = p * C + c;
         const V zk = lds[kx * ES + l];
-        const V zm = lds[((N - kx) & (N - 1)) * ES + l];
+        const V zm = lds[(kx == 0 ? 0 : N - kx) * ES + l];
         V r;
         if (ab == 0) {
             r.x = (T)0.5 * (zk.x + zm.x);
@@ -586,6 +626,13 @@ static int launch_c2c(const C2CArgs<T> &A, int pass, int S, hipStream_t st) {
         case 256: { constexpr int NN = 256; return CALL; }       \
         case 512: { constexpr int NN = 512; return CALL; }       \
         case 1024: { constexpr int NN = 1024; return CALL; }     \
+        case 12: { constexpr int NN = 12; return CALL; }         \
+        case 24: { constexpr int NN = 24; return CALL; }         \
+        case 48: { constexpr int NN = 48; return CALL; }         \
+        case 96: { constexpr int NN = 96; return CALL; }         \
+        case 192: { constexpr int NN = 192; return CALL; }       \
+        case 384: { constexpr int NN = 384; return CALL; }       \
+        case 768: { constexpr int NN = 768; return CALL; }       \
         default: return fail(LF2MI_ERANGE, "FFT size %d not supported", N_); \
     }
 
@@ -835,7 +882,9 @@ extern "C" {
 static int plan_create(int N, int C, int rhalo, int khalo, int dtype, int c2c, lf2mi_fft_plan *plan) {
     if (!plan) return fail(LF2MI_EINVAL, "lf2mi_fft_plan_create: null plan pointer");
     *plan = nullptr;
-    if (!is_pow2(N) || N < 8 || N > 1024) return fail(LF2MI_ERANGE, "FFT size N=%d: a power of two in [8, 1024] is required", N);
+    const bool pow2 = is_pow2(N), three = N % 3 == 0 && is_pow2(N / 3);
+    if (!(pow2 || three) || N < 8 || N > 1024)
+        return fail(LF2MI_ERANGE, "FFT size N=%d: a power of two or three times one, in [8, 1024], is required", N);
     if (C < 1 || rhalo < 0 || khalo < 0) return fail(LF2MI_EINVAL, "bad plan geometry C=%d rhalo=%d khalo=%d", C, rhalo, khalo);
     if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
     // components one x-pass tile holds (Tune<T>::x_L_max, and c2c_x_L for C2C plans)

@@ -38,7 +38,9 @@ def test_python_binding_covers_the_header():
 
 
 @pytest.mark.parametrize("call,args,code", [
-    ("lf2mi_fft_plan_create", (12, 1, 1, 0, 0), -4),      # N not a power of two
+    ("lf2mi_fft_plan_create", (10, 1, 1, 0, 0), -4),      # N neither 2^k nor 3 * 2^k
+    ("lf2mi_fft_plan_create", (36, 1, 1, 0, 0), -4),      # 9 * 2^2: no radix-9 plan
+    ("lf2mi_fft_plan_create", (6, 1, 1, 0, 0), -4),       # below the supported range
     ("lf2mi_fft_plan_create", (2048, 1, 1, 0, 0), -4),    # above the supported range
     ("lf2mi_fft_plan_create", (64, 0, 1, 0, 0), -1),      # no components
     ("lf2mi_fft_plan_create", (64, 1, 1, 0, 7), -1),      # bad precision

@@ -85,7 +85,7 @@ def test_pm_step_c1_uniform_forces(prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 1e-5)])
-@pytest.mark.parametrize("N", [64, 256])
+@pytest.mark.parametrize("N", [64, 96, 256])
 def test_pm_step_lattice_gaussian(N, prec, tol):
     """the kick-drift configuration (256^3 mesh with 256^3 particles is its full size): grid +
     N(0, 0.3 cells), v ~ N(0, 0.01), dt 0.01, mass 1/Np (mean density 1)"""
