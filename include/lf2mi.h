@@ -124,8 +124,8 @@ int lf2mi_field_affine(void *field, int N, int halo, int C, int comp, double mul
 typedef struct lf2mi_fft_plan_s *lf2mi_fft_plan;
 
 /* PlanFFT<Imag>(Field<Real>*, Field<Imag>*) for the R2C case
- * (LATfield2_PlanFFT_decl.hpp:731-846): N in [8, 1024], a power of two or three times
- * one (12, 24, 48, 96, 192, 384, 768), C >= 1
+ * (LATfield2_PlanFFT_decl.hpp:731-846): N in [8, 1024], 2^k, 3 * 2^k (12 ... 768) or
+ * 5 * 2^k (10 ... 640), C >= 1
  * components, real ghost width rhalo, k ghost width khalo.  Owns its twiddle tables
  * and workspace on the current device. */
 int lf2mi_fft_plan_create(int N, int C, int rhalo, int khalo, int dtype, lf2mi_fft_plan *plan);

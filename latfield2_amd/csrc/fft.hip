@@ -86,9 +86,34 @@ template <int S, typename V> __device__ __forceinline__ void dft3(V &a0, V &a1, 
     a2 = csub(t2, t3);
 }
 
+// y_k = sum_j a_j w^(jk), w = exp(S 2 pi i / 5): the pairs (a1, a4) and (a2, a3) combine
+// through cos and sin of 2 pi / 5 and 4 pi / 5
+template <int S, typename V> __device__ __forceinline__ void dft5(V *a) {
+    using T = decltype(a[0].x);
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    const V t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]), t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+    V b1, b2, e1, e2;
+    b1.x = a[0].x + c1 * t1.x + c2 * t2.x;
+    b1.y = a[0].y + c1 * t1.y + c2 * t2.y;
+    b2.x = a[0].x + c2 * t1.x + c1 * t2.x;
+    b2.y = a[0].y + c2 * t1.y + c1 * t2.y;
+    e1.x = s1 * t3.x + s2 * t4.x;
+    e1.y = s1 * t3.y + s2 * t4.y;
+    e2.x = s2 * t3.x - s1 * t4.x;
+    e2.y = s2 * t3.y - s1 * t4.y;
+    const V d1 = mul_si<S>(e1), d2 = mul_si<S>(e2);
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(b1, d1);
+    a[4] = csub(b1, d1);
+    a[2] = cadd(b2, d2);
+    a[3] = csub(b2, d2);
+}
+
 template <int R, int S, typename V> __device__ __forceinline__ void dft(V *a) {
     if constexpr (R == 2) dft2<S>(a[0], a[1]);
     else if constexpr (R == 3) dft3<S>(a[0], a[1], a[2]);
+    else if constexpr (R == 5) dft5<S>(a);
     else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
     else dft8<S>(a);
 }
@@ -98,11 +123,14 @@ template <int R, int S, typename V> __device__ __forceinline__ void dft(V *a) {
 //   powers of two: 8 = [8], 16 = [4,4], 32 = [8,4], 64 = [8,8], 128 = [8,4,4], 256 = [8,8,4],
 //                  512 = [8,8,8], 1024 = [8,8,4,4];
 //   3 * 2^k:       12 = [4,3], 24 = [8,3], 48 = [4,4,3], 96 = [8,4,3], 192 = [8,8,3],
-//                  384 = [8,4,4,3], 768 = [8,8,4,3].
+//                  384 = [8,4,4,3], 768 = [8,8,4,3];
+//   5 * 2^k:       10 = [2,5], 20 = [4,5], 40 = [8,5], 80 = [4,4,5], 160 = [8,4,5],
+//                  320 = [8,8,5], 640 = [8,4,4,5].
 constexpr int fft_np(int N) {
     return N == 8 ? 1
-         : (N == 16 || N == 32 || N == 64 || N == 12 || N == 24) ? 2
-         : (N == 128 || N == 256 || N == 512 || N == 48 || N == 96 || N == 192) ? 3
+         : (N == 16 || N == 32 || N == 64 || N == 12 || N == 24 || N == 10 || N == 20 || N == 40) ? 2
+         : (N == 128 || N == 256 || N == 512 || N == 48 || N == 96 || N == 192 || N == 80 || N == 160 ||
+            N == 320) ? 3
          : 4;
 }
 constexpr int fft_radix(int N, int p) {
@@ -118,6 +146,13 @@ constexpr int fft_radix(int N, int p) {
          : N == 192 ? (p < 2 ? 8 : 3)
          : N == 384 ? (p == 0 ? 8 : (p < 3 ? 4 : 3))
          : N == 768 ? (p < 2 ? 8 : (p == 2 ? 4 : 3))
+         : N == 10 ? (p == 0 ? 2 : 5)
+         : N == 20 ? (p == 0 ? 4 : 5)
+         : N == 40 ? (p == 0 ? 8 : 5)
+         : N == 80 ? (p < 2 ? 4 : 5)
+         : N == 160 ? (p == 0 ? 8 : (p == 1 ? 4 : 5))
+         : N == 320 ? (p < 2 ? 8 : 5)
+         : N == 640 ? (p == 0 ? 8 : (p < 3 ? 4 : 5))
          : 8;
 }
 // the product of the plan's radices is N
@@ -126,7 +161,9 @@ static_assert(fft_prod(8, 0) == 8 && fft_prod(16, 0) == 16 && fft_prod(32, 0) ==
                   fft_prod(128, 0) == 128 && fft_prod(256, 0) == 256 && fft_prod(512, 0) == 512 &&
                   fft_prod(1024, 0) == 1024 && fft_prod(12, 0) == 12 && fft_prod(24, 0) == 24 &&
                   fft_prod(48, 0) == 48 && fft_prod(96, 0) == 96 && fft_prod(192, 0) == 192 &&
-                  fft_prod(384, 0) == 384 && fft_prod(768, 0) == 768,
+                  fft_prod(384, 0) == 384 && fft_prod(768, 0) == 768 && fft_prod(10, 0) == 10 &&
+                  fft_prod(20, 0) == 20 && fft_prod(40, 0) == 40 && fft_prod(80, 0) == 80 &&
+                  fft_prod(160, 0) == 160 && fft_prod(320, 0) == 320 && fft_prod(640, 0) == 640,
               "radix plans");
 constexpr int fft_ns(int N, int p) { return p == 0 ? 1 : fft_ns(N, p - 1) * fft_radix(N, p - 1); }
 
@@ -633,6 +670,13 @@ static int launch_c2c(const C2CArgs<T> &A, int pass, int S, hipStream_t st) {
         case 192: { constexpr int NN = 192; return CALL; }       \
         case 384: { constexpr int NN = 384; return CALL; }       \
         case 768: { constexpr int NN = 768; return CALL; }       \
+        case 10: { constexpr int NN = 10; return CALL; }         \
+        case 20: { constexpr int NN = 20; return CALL; }         \
+        case 40: { constexpr int NN = 40; return CALL; }         \
+        case 80: { constexpr int NN = 80; return CALL; }         \
+        case 160: { constexpr int NN = 160; return CALL; }       \
+        case 320: { constexpr int NN = 320; return CALL; }       \
+        case 640: { constexpr int NN = 640; return CALL; }       \
         default: return fail(LF2MI_ERANGE, "FFT size %d not supported", N_); \
     }
 
@@ -882,9 +926,9 @@ extern "C" {
 static int plan_create(int N, int C, int rhalo, int khalo, int dtype, int c2c, lf2mi_fft_plan *plan) {
     if (!plan) return fail(LF2MI_EINVAL, "lf2mi_fft_plan_create: null plan pointer");
     *plan = nullptr;
-    const bool pow2 = is_pow2(N), three = N % 3 == 0 && is_pow2(N / 3);
-    if (!(pow2 || three) || N < 8 || N > 1024)
-        return fail(LF2MI_ERANGE, "FFT size N=%d: a power of two or three times one, in [8, 1024], is required", N);
+    const bool pow2 = is_pow2(N), three = N % 3 == 0 && is_pow2(N / 3), five = N % 5 == 0 && is_pow2(N / 5);
+    if (!(pow2 || three || five) || N < 8 || N > 1024)
+        return fail(LF2MI_ERANGE, "FFT size N=%d: 2^k, 3 * 2^k or 5 * 2^k in [8, 1024] is required", N);
     if (C < 1 || rhalo < 0 || khalo < 0) return fail(LF2MI_EINVAL, "bad plan geometry C=%d rhalo=%d khalo=%d", C, rhalo, khalo);
     if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
     // components one x-pass tile holds (Tune<T>::x_L_max, and c2c_x_L for C2C plans)

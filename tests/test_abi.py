@@ -38,7 +38,7 @@ def test_python_binding_covers_the_header():
 
 
 @pytest.mark.parametrize("call,args,code", [
-    ("lf2mi_fft_plan_create", (10, 1, 1, 0, 0), -4),      # N neither 2^k nor 3 * 2^k
+    ("lf2mi_fft_plan_create", (14, 1, 1, 0, 0), -4),      # N not 2^k, 3 * 2^k or 5 * 2^k
     ("lf2mi_fft_plan_create", (36, 1, 1, 0, 0), -4),      # 9 * 2^2: no radix-9 plan
     ("lf2mi_fft_plan_create", (6, 1, 1, 0, 0), -4),       # below the supported range
     ("lf2mi_fft_plan_create", (2048, 1, 1, 0, 0), -4),    # above the supported range

@@ -74,8 +74,9 @@ def _fft_case(N, h, kh, C, prec, seed):
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
 @pytest.mark.parametrize("N,h,kh,C", [(8, 1, 0, 1), (16, 1, 0, 1), (32, 2, 1, 3), (64, 1, 0, 1), (128, 1, 0, 1),
                                       (16, 1, 2, 6),
-                                      # three times a power of two (radix-3 last pass)
-                                      (12, 1, 0, 1), (24, 1, 1, 2), (96, 1, 0, 1), (192, 2, 1, 1)])
+                                      # three or five times a power of two (radix-3/5 last pass)
+                                      (12, 1, 0, 1), (24, 1, 1, 2), (96, 1, 0, 1), (192, 2, 1, 1),
+                                      (10, 1, 0, 1), (40, 1, 1, 2), (80, 1, 0, 1), (160, 1, 0, 1)])
 def test_fft_forward_matches_oracle(prec, tol, N, h, kh, C):
     announce(f"fft forward {prec} N={N} h={h} kh={kh} C={C}")
     r, k, plan = _fft_case(N, h, kh, C, prec, seed=N + C)
@@ -93,7 +94,7 @@ def test_fft_forward_matches_oracle(prec, tol, N, h, kh, C):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
 @pytest.mark.parametrize("N,h,kh,C", [(8, 1, 0, 1), (32, 1, 1, 1), (64, 2, 0, 3), (256, 1, 0, 1),
-                                      (12, 1, 0, 1), (48, 1, 1, 3), (96, 1, 0, 1)])
+                                      (12, 1, 0, 1), (48, 1, 1, 3), (96, 1, 0, 1), (20, 1, 0, 1), (160, 1, 1, 1)])
 def test_fft_backward_matches_oracle(prec, tol, N, h, kh, C):
     announce(f"fft backward {prec} N={N} h={h} kh={kh} C={C}")
     lat = lf.Lattice(3, N, h)
@@ -133,7 +134,7 @@ def _close(got, ref, prec, tol):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
 @pytest.mark.parametrize("N,h,kh,C", [(8, 1, 1, 1), (16, 0, 1, 1), (32, 2, 1, 3), (64, 1, 0, 1), (128, 1, 1, 1),
-                                      (16, 1, 2, 6), (12, 1, 1, 1), (96, 1, 0, 2)])
+                                      (16, 1, 2, 6), (12, 1, 1, 1), (96, 1, 0, 2), (40, 1, 1, 1)])
 def test_c2c_forward_matches_oracle(prec, tol, N, h, kh, C):
     """PlanFFT<Imag>(Field<Imag>*, Field<Imag>*) forward (LATfield2_PlanFFT_decl.hpp:1061-1133)"""
     announce(f"c2c forward {prec} N={N} h={h} kh={kh} C={C}")
@@ -225,10 +226,11 @@ def test_fp64_round_trips_at_1024(c2c):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("N", [384, 768])
+@pytest.mark.parametrize("N", [384, 768, 320, 640])
 @pytest.mark.parametrize("c2c", [False, True])
 def test_round_trips_at_three_times_a_power_of_two(N, c2c):
-    """R2C/C2R and C2C at 384^3 and 768^3 in FP32 (radix plans [8,4,4,3], [8,8,4,3]):
+    """R2C/C2R and C2C at 384^3, 768^3 (radix plans [8,4,4,3], [8,8,4,3]) and 320^3, 640^3
+    ([8,8,5], [8,4,4,5]) in FP32:
     backward(forward(f)) / N^3 = f to 2e-6 rel-L2 on the device"""
     announce(f"round trip N={N} c2c={c2c}")
     lat = lf.Lattice(3, N, 1)
@@ -251,7 +253,7 @@ def test_round_trips_at_three_times_a_power_of_two(N, c2c):
 
 def test_fft_sizes_outside_the_plans_are_rejected():
     announce("fft size checks")
-    for N in (6, 10, 20, 36):
+    for N in (6, 14, 36, 90):
         with pytest.raises(lf.Lf2miError):
             lf.PlanFFT(lf.Field(lf.Lattice(3, N, 1), 1, lf.F32), None)
 
@@ -311,7 +313,7 @@ def test_poisson_green_matches_oracle(prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
-@pytest.mark.parametrize("N", [16, 64, 128, 96])
+@pytest.mark.parametrize("N", [16, 64, 128, 96, 80])
 def test_fused_poisson_solve_matches_oracle_chain(prec, tol, N):
     """mean-subtract -> FFT -> -1/K^2 (zero mode 0) -> inverse FFT / N^3, as in orc_pm_step"""
     announce(f"poisson_solve {prec} N={N}")
