@@ -14,6 +14,8 @@ Array conventions
   ``[z, y, x, c]`` with ``G = N + 2h`` (storage order of ``Field<Real>``);
 * k-fields: complex128 arrays of shape ``(N+2kh, N+2kh, N/2+1+2kh, C)`` indexed
   ``[ky, kz, kx, c]`` (storage order of the R2C ``Field<Imag>``);
+* complex fields of the C2C transform: complex128 arrays ``(G, G, G, C)``, position space
+  ``[z, y, x, c]`` and k space ``[ky, kx, kz, c]`` (storage order read by ``cKSite``);
 * particles: ``pos``/``vel`` float64 arrays of shape ``(np, 3)`` in box units.
 """
 from __future__ import annotations
