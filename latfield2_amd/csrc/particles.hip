@@ -1524,19 +1524,40 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
         }
         // GKD_STAGE loads in flight per lane before their LDS stores (a load-store loop
         // would wait out one HBM round trip per element)
-        const int nw = wx * wy * wy;
-        for (int i0 = threadIdx.x; i0 < nw; i0 += GKD_STAGE * DEP_WG) {
-            float v[GKD_STAGE];
+        // rows (y, z) of the window, one wave per row: the row's base index is wave-uniform,
+        // a lane holds x = lane and x = lane + 64
+        {
+            const int nrows = wy * wy;
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int lane = threadIdx.x & 63;
+            const int64_t G = (int64_t)N + 2 * A.h;
+            const bool has0 = lane < wx, has1 = lane + 64 < wx;
+            const int gx0 = wrapi(ox - 1 + lane, N) + A.h;
+            const int gx1 = has1 ? wrapi(ox - 1 + 64 + lane, N) + A.h : 0;
+            constexpr int NWV = DEP_WG / 64;
+            for (int r0 = wave; r0 < nrows; r0 += NWV * GKD_STAGE) {
+                float v0[GKD_STAGE], v1[GKD_STAGE];
 #pragma unroll
-            for (int k = 0; k < GKD_STAGE; ++k) {
-                const int i = i0 + k * DEP_WG;
-                const int x = i % wx, y = (i / wx) % wy, z = i / (wx * wy);
-                v[k] = i < nw ? A.phi[site_index(N, A.h, wrapi(ox - 1 + x, N), wrapi(oy - 1 + y, N), wrapi(oz - 1 + z, N))]
-                              : 0.0f;
+                for (int k = 0; k < GKD_STAGE; ++k) {
+                    const int row = r0 + NWV * k;
+                    v0[k] = 0.0f;
+                    v1[k] = 0.0f;
+                    if (row < nrows) {
+                        const int y = row % wy, z = row / wy;
+                        const int64_t rb = G * ((int64_t)(wrapi(oy - 1 + y, N) + A.h) + G * (int64_t)(wrapi(oz - 1 + z, N) + A.h));
+                        if (has0) v0[k] = A.phi[rb + gx0];
+                        if (has1) v1[k] = A.phi[rb + gx1];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < GKD_STAGE; ++k) {
+                    const int row = r0 + NWV * k;
+                    if (row < nrows) {
+                        if (has0) win[lane + wx * row] = v0[k];
+                        if (has1) win[lane + 64 + wx * row] = v1[k];
+                    }
+                }
             }
-#pragma unroll
-            for (int k = 0; k < GKD_STAGE; ++k)
-                if (i0 + k * DEP_WG < nw) win[i0 + k * DEP_WG] = v[k];
         }
         __syncthreads();
         for (; p < ch.y; p += DEP_WG) {
@@ -1561,7 +1582,10 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
             nx /= B;
             ny /= B;
             nz /= B;
-            int dx = (nx - bx0 + NB) % NB, dy = (ny - by + NB) % NB, dz = (nz - bz + NB) % NB;
+            int dx = nx - bx0, dy = ny - by, dz = nz - bz;
+            dx += dx < 0 ? NB : 0;
+            dy += dy < 0 ? NB : 0;
+            dz += dz < 0 ? NB : 0;
             if (dx > nbx) dx -= NB;
             if (dy > 1) dy -= NB;
             if (dz > 1) dz -= NB;
