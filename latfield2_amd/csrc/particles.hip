@@ -1446,7 +1446,11 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
 // per chunk rather than once per particle stencil (the L1 gathers of the per-particle
 // path are bound by the number of lines they touch).
 static constexpr int GKD_CHUNK_B = 8;
-static constexpr int GKD_STAGE = 8;
+// window rows a wave keeps in flight: with 4 waves x 32 >= (MAXB + 3)^2 rows the whole window
+// is requested at once (a chunk is a chain of dependent latency phases, so each batch of
+// rows costs a full HBM round trip: 4 batches of 8 rows took 0.70 ms, one batch 0.63 ms)
+static constexpr int GKD_STAGE = 32;
+static_assert((DEP_WG / 64) * GKD_STAGE >= (MAXB + 3) * (MAXB + 3), "one batch of window rows");
 
 // walk one brick row, calling emit(p0, p1, bx0, nbx) per chunk
 template <typename F>
@@ -1522,10 +1526,9 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
             u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
             vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
         }
-        // GKD_STAGE loads in flight per lane before their LDS stores (a load-store loop
-        // would wait out one HBM round trip per element)
         // rows (y, z) of the window, one wave per row: the row's base index is wave-uniform,
-        // a lane holds x = lane and x = lane + 64
+        // a lane holds x = lane and x = lane + 64; every row's loads are issued before the
+        // LDS stores (a load-store loop would wait out one HBM round trip per row)
         {
             const int nrows = wy * wy;
             const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
