@@ -1383,11 +1383,27 @@ __global__ __launch_bounds__(DEP_WG) void gkd_f32_kernel(GkdArgs A) {
             u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
             vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
         }
-        for (int i = threadIdx.x; i < pw * pw * pw; i += DEP_WG) {
-            const int gx = wrapi(r.ox - 1 + i % pw, N);
-            const int gy = wrapi(r.oy - 1 + (i / pw) % pw, N);
-            const int gz = wrapi(r.oz - 1 + i / (pw * pw), N);
-            phw[i] = A.phi[site_index(N, A.h, gx, gy, gz)];
+        {
+            // every window load issued before the LDS stores
+            constexpr int WK = ((MAXB + 3) * (MAXB + 3) * (MAXB + 3) + DEP_WG - 1) / DEP_WG;
+            const int nwin = pw * pw * pw;
+            float wv[WK];
+#pragma unroll
+            for (int k = 0; k < WK; ++k) {
+                const int i = threadIdx.x + k * DEP_WG;
+                wv[k] = 0.0f;
+                if (i < nwin) {
+                    const int gx = wrapi(r.ox - 1 + i % pw, N);
+                    const int gy = wrapi(r.oy - 1 + (i / pw) % pw, N);
+                    const int gz = wrapi(r.oz - 1 + i / (pw * pw), N);
+                    wv[k] = A.phi[site_index(N, A.h, gx, gy, gz)];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < WK; ++k) {
+                const int i = threadIdx.x + k * DEP_WG;
+                if (i < nwin) phw[i] = wv[k];
+            }
         }
         __syncthreads();
         const float hn = 0.5f * (float)N;
@@ -1506,10 +1522,11 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
     float vmax = 0.0f;
     if (threadIdx.x == 0) red = 0u;
     // chunks are claimed one at a time from nchunk[1] (they hold 1 to GKD_CHUNK_B bricks)
+    // the following chunk is claimed while this one's particles are processed
+    if (threadIdx.x == 0) claim = atomicAdd(&nchunk[1], 1);
+    __syncthreads();
+    int ci = claim;
     for (;;) {
-        if (threadIdx.x == 0) claim = atomicAdd(&nchunk[1], 1);
-        __syncthreads();
-        const int ci = claim;
         if (ci >= total) break;
         const int4 ch = chunks[ci];
         const int b0 = ch.z, nbx = ch.w;
@@ -1563,6 +1580,7 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
             }
         }
         __syncthreads();
+        if (threadIdx.x == 0) claim = atomicAdd(&nchunk[1], 1);  // every lane has read ci
         for (; p < ch.y; p += DEP_WG) {
             if (p != ch.x + threadIdx.x) {
                 cell = A.s.cell[p];
@@ -1606,6 +1624,7 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
             atomicAdd(&A.next_counts[bx + NB * (bby + NB * bbz)], c);
         }
         __syncthreads();
+        ci = claim;
     }
     if (A.vmax2) {
         atomicMax(&red, __float_as_uint(vmax));
