@@ -283,8 +283,19 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
     using V = cplx<T>;
     extern __shared__ unsigned char smem_raw[];
     V *lds = reinterpret_cast<V *>(smem_raw);
-    const int col0 = blockIdx.x * L;
-    const int64_t batch = blockIdx.y;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if constexpr (L * (int)sizeof(V) < 128) {
+        // a tile row is part of a 128-byte line whose rest belongs to the next tiles: blocks are
+        // dispatched round-robin over the 8 XCDs, so give each XCD a contiguous range of tiles
+        // and the whole line one L2
+        const int n = gridDim.x * gridDim.y, lin = blockIdx.x + gridDim.x * blockIdx.y;
+        const int per = n / 8, rem = n % 8, xcd = lin % 8, slot = lin / 8;
+        const int t = xcd * per + min(xcd, rem) + slot;
+        bx = t % gridDim.x;
+        by = t / gridDim.x;
+    }
+    const int col0 = bx * L;
+    const int64_t batch = by;
     const V *in = A.in + A.in_col0 + batch * A.in_bs + col0;
     V *out = A.out + A.out_col0 + batch * A.out_bs + col0;
     const int ncols = A.ncols - col0;
