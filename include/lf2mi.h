@@ -100,6 +100,9 @@ int lf2mi_zero(void *ptr, int64_t count, int dtype, void *stream);
 /* Field::updateHalo: periodic ghost fill of all C components
  * (LATfield2_Field_decl.hpp:889-935); bit-exact copies */
 int lf2mi_ghost_fill(void *field, int N, int halo, int C, int dtype, void *stream);
+/* the same on an nx x ny x nz lattice (Field::updateHalo takes any 3-D lattice; the
+ * storage index is (x+h) + (nx+2h)((y+h) + (ny+2h)(z+h))); halo <= min(nx, ny, nz) */
+int lf2mi_ghost_fill_box(void *field, int nx, int ny, int nz, int halo, int C, int dtype, void *stream);
 
 /* the ghost folds after a deposit, in the reference's order (x, then y, then z):
  *   LF2MI_PROJ_SCALAR: scalarProjectionCIC_comm (particles/projections.hpp:194-300)
@@ -318,6 +321,32 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
  * NULL when dt_kick == 0: a drift only (move_particles_simple, move_function.hpp:30). */
 int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick,
                             double dt_drift, double *vmax2, void *stream);
+
+/* The output[] / reduce_type[] statistics of Particles::updateVel and moveParticles
+ * (particles/LATfield2_Particles.hpp:788-807, :846-885; reduction bits :81-86).  For each
+ * i < n, out[i] receives quantity[i] of every particle reduced by reduce_type[i]:
+ *   reduce_type & (SUM | SUM_LOCAL): the sum (starting from 0);
+ *   else & (MIN | MIN_LOCAL): the minimum, starting from 9223372036854775807;
+ *   else & (MAX | MAX_LOCAL): the maximum, starting from -9223372036854775807;
+ *   else out[i] is left as it is.  On one device the _LOCAL forms equal the global ones.
+ * Quantities, evaluated in FP64 from the container's state; m is the mass the deposits use
+ * (the global mass or the per-particle one, lf2mi_parts_mass_from): */
+#define LF2MI_PQ_V2 0   /* |v|^2 */
+#define LF2MI_PQ_MVX 1  /* m v_x */
+#define LF2MI_PQ_MVY 2  /* m v_y */
+#define LF2MI_PQ_MVZ 3  /* m v_z */
+#define LF2MI_PQ_MASS 4 /* m */
+#define LF2MI_PQ_EKIN 5 /* m |v|^2 / 2 */
+#define LF2MI_RED_SUM 1
+#define LF2MI_RED_MIN 2
+#define LF2MI_RED_MAX 4
+#define LF2MI_RED_SUM_LOCAL 8
+#define LF2MI_RED_MIN_LOCAL 16
+#define LF2MI_RED_MAX_LOCAL 32
+/* n <= 8; quantity, reduce_type and out are host arrays; the sums are formed in a fixed
+ * order, so they are the same from run to run.  Synchronises. */
+int lf2mi_parts_reduce(lf2mi_parts p, int n, const int *quantity, const int *reduce_type, double *out,
+                       void *stream);
 
 /* One PM step (the driver sequence of examples/poissonSolver.cpp plus the particle
  * update): zero rho, periodic CIC deposit, mean-subtracted Poisson solve with

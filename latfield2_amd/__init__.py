@@ -21,8 +21,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART, MASS_PART_F32,
-                   FFT_BACKWARD, FFT_FORWARD, PARTS_ID, PARTS_MASS, PROJ_SCALAR, PROJ_SYMTENSOR_NGP,
-                   PROJ_VECTOR, PROJ_VECTOR_NGP, PROJ_VECVEC, Lf2miError, call, load)
+                   FFT_BACKWARD, FFT_FORWARD, MAX, MAX_LOCAL, MIN, MIN_LOCAL, PARTS_ID, PARTS_MASS, PQ_EKIN,
+                   PQ_MASS, PQ_MVX, PQ_MVY, PQ_MVZ, PQ_V2, PROJ_SCALAR, PROJ_SYMTENSOR_NGP, PROJ_VECTOR,
+                   PROJ_VECTOR_NGP, PROJ_VECVEC, SUM, SUM_LOCAL, Lf2miError, call, load)
 
 __all__ = [
     "Lattice", "Field", "KField", "CField", "PlanFFT", "Particles", "projection_init", "bin_positions",
@@ -30,7 +31,8 @@ __all__ = [
     "vectorProjectionCIC_comm", "VecVecProjectionCIC_project", "VecVecProjectionCIC_comm",
     "vectorProjectionCICNGP_project", "vectorProjectionCICNGP_comm", "symtensorProjectionCICNGP_project",
     "symtensorProjectionCICNGP_comm", "poisson_green", "poisson_solve", "pm_step", "FFT_FORWARD", "FFT_BACKWARD", "F32", "F64",
-    "Lf2miError", "load",
+    "Lf2miError", "load", "updateVel_simple", "updateVel_cicKick", "move_particles_simple",
+    "SUM", "MIN", "MAX", "SUM_LOCAL", "MIN_LOCAL", "MAX_LOCAL",
 ]
 
 FROM_INFO = 0  # particles/projections.hpp:39-40 (global mass)
@@ -240,6 +242,47 @@ def _dp(a):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+class _DeviceFunctor:
+    """A particle functor of the reference's updateVel / moveParticles interface
+    (particles/LATfield2_Particles.hpp:217-244) that runs on the device.  Its per-particle
+    outputs are, by slot: |v|^2, m v_x, m v_y, m v_z, m, m |v|^2 / 2 after the update."""
+
+    def __init__(self, name: str, kind: str):
+        self.name, self.kind = name, kind
+
+    def __call__(self, *args, **kwargs):
+        raise Lf2miError(f"{self.name} runs on the device: pass it to Particles.{self.kind}")
+
+    def __repr__(self):
+        return f"<device functor {self.name}>"
+
+
+# updateVel_function.hpp:13-38 (the identity on the velocities)
+updateVel_simple = _DeviceFunctor("updateVel_simple", "updateVel")
+# the PM gather + kick: v += dtau * CIC(-grad phi) of the potential fields[0]
+updateVel_cicKick = _DeviceFunctor("updateVel_cicKick", "updateVel")
+# move_function.hpp:15-32: pos += dtau * vel, wrapped and re-binned
+move_particles_simple = _DeviceFunctor("move_particles_simple", "moveParticles")
+_OUTPUT_SLOTS = (PQ_V2, PQ_MVX, PQ_MVY, PQ_MVZ, PQ_MASS, PQ_EKIN)
+
+
+def _reduce_outputs(parts, output, reduce_type, noutput):
+    """max |v|^2 and the functor outputs of slots 0..noutput-1, reduced by reduce_type
+    (particles/LATfield2_Particles.hpp:788-807, :846-885)"""
+    if noutput > len(_OUTPUT_SLOTS):
+        raise Lf2miError(f"the device functors have {len(_OUTPUT_SLOTS)} output slots")
+    if noutput and (output is None or reduce_type is None):
+        raise Lf2miError("output and reduce_type are needed for noutput > 0")
+    q = [PQ_V2] + list(_OUTPUT_SLOTS[:noutput])
+    rt = [MAX] + [int(reduce_type[i]) for i in range(noutput)]
+    init = [0.0] + [float(output[i]) for i in range(noutput)]
+    res = parts.reduce(q, rt, init)
+    for i in range(noutput):
+        output[i] = res[i + 1]
+    return res[0] if parts.count() > 0 else 0.0
+
+
+
 class Particles:
     """Particles<part_simple, part_simple_info, ...> on the device (box length 1)."""
 
@@ -396,6 +439,39 @@ class Particles:
              float(dt_kick), float(dt_drift),
              ctypes.byref(vm) if want_vmax else None, _stream())
         return vm.value if want_vmax else None
+
+    def updateVel(self, funct, dtau, fields=None, nfields=0, params=None, output=None, reduce_type=None,
+                             noutput=0):
+        """Particles::updateVel (particles/LATfield2_Particles.hpp:753-893) with a device functor;
+        returns sqrt(max |v|^2) after the update and fills output[:noutput]"""
+        if funct is updateVel_cicKick:
+            if not fields or fields[0].C != 1:
+                raise Lf2miError("updateVel_cicKick needs the potential as fields[0]")
+            self.gather_kick_drift(fields[0], float(dtau), 0.0)
+        elif funct is not updateVel_simple:
+            raise Lf2miError("this functor has no device implementation (updateVel_simple and updateVel_cicKick have)")
+        return float(np.sqrt(_reduce_outputs(self, output, reduce_type, noutput)))
+
+
+    def moveParticles(self, funct, dtau, fields=None, nfields=0, params=None, output=None, reduce_type=None,
+                                 noutput=0):
+        """Particles::moveParticles (particles/LATfield2_Particles.hpp:1811-2803) with a device functor"""
+        if funct is not move_particles_simple:
+            raise Lf2miError("this functor has no device implementation (move_particles_simple has)")
+        self.gather_kick_drift(None, 0.0, float(dtau))
+        if noutput:
+            _reduce_outputs(self, output, reduce_type, noutput)
+
+
+    def reduce(self, quantities, reduce_types, init=None):
+        """lf2mi_parts_reduce: FP64 statistics of the particles (PQ_* quantities, SUM/MIN/MAX
+        reduction bits); a slot whose type has none of the bits keeps its `init` value"""
+        n = len(quantities)
+        q = (ctypes.c_int32 * n)(*[int(v) for v in quantities])
+        rt = (ctypes.c_int32 * n)(*[int(v) for v in reduce_types])
+        out = (ctypes.c_double * n)(*([0.0] * n if init is None else [float(v) for v in init]))
+        call("lf2mi_parts_reduce", self.h, n, q, rt, out, _stream())
+        return list(out)
 
 
 def bin_positions(pos, N: int, want_keys: bool = True, want_counts: bool = True):

@@ -20,6 +20,9 @@ DEP_UNFOLDED, DEP_PERIODIC = 0, 1
 ACC_NATIVE, ACC_FIXED64 = 0, 1
 PARTS_ID, PARTS_MASS = 1, 2
 MASS_INFO, MASS_PART, MASS_PART_F32 = 0, 1, 2
+# updateVel / moveParticles statistics (particles/LATfield2_Particles.hpp:81-86)
+PQ_V2, PQ_MVX, PQ_MVY, PQ_MVZ, PQ_MASS, PQ_EKIN = 0, 1, 2, 3, 4, 5
+SUM, MIN, MAX, SUM_LOCAL, MIN_LOCAL, MAX_LOCAL = 1, 2, 4, 8, 16, 32
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -44,6 +47,7 @@ SIGNATURES = {
     "lf2mi_profile_get": [_i, ctypes.c_char_p, _i, _dp, _i64p],
     "lf2mi_zero": [_vp, _i64, _i, _vp],
     "lf2mi_ghost_fill": [_vp, _i, _i, _i, _i, _vp],
+    "lf2mi_ghost_fill_box": [_vp, _i, _i, _i, _i, _i, _i, _vp],
     "lf2mi_projection_comm": [_vp, _i, _i, _i, _i, _vp],
     "lf2mi_field_sum": [_vp, _i, _i, _i, _i, _i, _dp, _vp],
     "lf2mi_field_affine": [_vp, _i, _i, _i, _i, _d, _d, _i, _vp],
@@ -74,6 +78,7 @@ SIGNATURES = {
     "lf2mi_parts_ic_blobs": [_vp, _i64, _i, _d, _d, _d, _d, _u64, _vp],
     "lf2mi_deposit": [_vp, _i, _vp, _i, _i, _i, _vp],
     "lf2mi_gather_kick_drift": [_vp, _vp, _i, _d, _d, _dp, _vp],
+    "lf2mi_parts_reduce": [_vp, _i, _i32p, _i32p, _dp, _vp],
     "lf2mi_pm_step": [_vp, _vp, _vp, _vp, _i, _d, _d, _i, _vp],
     "lf2mi_diffpm_create": [_i, _i64, _i, _d, _d, _i, ctypes.POINTER(_vp)],
     "lf2mi_diffpm_destroy": [_vp],

@@ -737,24 +737,34 @@ static int forward_x_only(lf2mi_fft_plan_s *p, const void *rfield, cplx<T> *kout
     return xpass<T>(p->N, X, -1, st);
 }
 
+// The forward transform goes through the workspace: the x pass reads the real field into W,
+// the z pass works in place on W and the y pass writes the k-field.  Every read of the real
+// field thus precedes every write of the k-field, so the two may share storage
+// (FFT_IN_PLACE, LATfield2_PlanFFT_decl.hpp:804-824); the bytes moved are the same.
 template <typename T>
 static int forward(lf2mi_fft_plan_s *p, const void *rfield, cplx<T> *kout, KGeo kg, T shift, hipStream_t st) {
     const int N = p->N;
-    if (int rc = forward_x_only<T>(p, rfield, kout, kg, shift, st)) return rc;
+    cplx<T> *W = (cplx<T> *)p->work;
+    const KGeo wg = work_geo(p);
+    if (int rc = forward_x_only<T>(p, rfield, W, wg, shift, st)) return rc;
     StridedArgs<T> A{};
-    A.in = kout;
-    A.out = kout;
-    A.in_col0 = A.out_col0 = kg.col0;
+    A.in = W;
+    A.out = W;
+    A.in_col0 = A.out_col0 = wg.col0;
     A.ncols = (N / 2 + 1) * p->C;
     A.tw = (const cplx<T> *)p->tw;
     A.C = p->C;
-    // z pass: along slot1, batches over slot2
-    A.in_es = A.out_es = kg.s1;
-    A.in_bs = A.out_bs = kg.s2;
+    // z pass in place on W: along slot1, batches over slot2
+    A.in_es = A.out_es = wg.s1;
+    A.in_bs = A.out_bs = wg.s2;
     if (int rc = strided<T>(N, A, -1, 0, N, st)) return rc;
-    // y pass: along slot2, batches over slot1
-    A.in_es = A.out_es = kg.s2;
-    A.in_bs = A.out_bs = kg.s1;
+    // y pass W -> k-field: along slot2, batches over slot1
+    A.out = kout;
+    A.out_col0 = kg.col0;
+    A.in_es = wg.s2;
+    A.out_es = kg.s2;
+    A.in_bs = wg.s1;
+    A.out_bs = kg.s1;
     return strided<T>(N, A, -1, 0, N, st);
 }
 

@@ -100,43 +100,45 @@ __global__ void zero_kernel(T *__restrict__ p, int64_t n) {
 // interior).  Each copies from its periodic image, so the result does not depend on
 // the order (LATfield2_Field_decl.hpp:905-926 computes the same source index).
 template <typename T>
-__global__ void ghost_fill_kernel(T *__restrict__ f, int N, int h, int C) {
-    const int G = N + 2 * h;
-    const int64_t nz = 2LL * h * G * G;          // z-slabs
-    const int64_t ny = 2LL * h * N * G;          // y-slabs, z interior
-    const int64_t nx = 2LL * h * N * N;          // x-slabs, y,z interior
-    const int64_t total = (nz + ny + nx) * C;
+__global__ void ghost_fill_kernel(T *__restrict__ f, int nx, int ny, int nz, int h, int C) {
+    const int Gx = nx + 2 * h, Gy = ny + 2 * h;
+    const int64_t cz = 2LL * h * Gx * Gy;        // z-slabs
+    const int64_t cy = 2LL * h * nz * Gx;        // y-slabs, z interior
+    const int64_t cx = 2LL * h * ny * nz;        // x-slabs, y,z interior
+    const int64_t total = (cz + cy + cx) * C;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
         const int c = (int)(t % C);
         int64_t s = t / C;
-        int x, y, z;  // interior coordinates in [-h, N+h)
-        if (s < nz) {
-            x = (int)(s % G) - h;
-            s /= G;
-            y = (int)(s % G) - h;
-            s /= G;
+        int x, y, z;  // interior coordinates in [-h, n+h)
+        if (s < cz) {
+            x = (int)(s % Gx) - h;
+            s /= Gx;
+            y = (int)(s % Gy) - h;
+            s /= Gy;
             const int k = (int)s;  // [0, 2h)
-            z = k < h ? k - h : N + (k - h);
-        } else if ((s -= nz) < ny) {
-            x = (int)(s % G) - h;
-            s /= G;
+            z = k < h ? k - h : nz + (k - h);
+        } else if ((s -= cz) < cy) {
+            x = (int)(s % Gx) - h;
+            s /= Gx;
             const int k = (int)(s % (2 * h));
             s /= (2 * h);
             z = (int)s;
-            y = k < h ? k - h : N + (k - h);
+            y = k < h ? k - h : ny + (k - h);
         } else {
-            s -= ny;
+            s -= cy;
             const int k = (int)(s % (2 * h));
             s /= (2 * h);
-            y = (int)(s % N);
-            z = (int)(s / N);
-            x = k < h ? k - h : N + (k - h);
+            y = (int)(s % ny);
+            z = (int)(s / ny);
+            x = k < h ? k - h : nx + (k - h);
         }
-        const int sx = x < 0 ? x + N : (x >= N ? x - N : x);
-        const int sy = y < 0 ? y + N : (y >= N ? y - N : y);
-        const int sz = z < 0 ? z + N : (z >= N ? z - N : z);
-        f[site_index(N, h, x, y, z) * C + c] = f[site_index(N, h, sx, sy, sz) * C + c];
+        const int sx = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        const int sy = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        const int sz = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        const int64_t dst = (int64_t)(x + h) + Gx * ((int64_t)(y + h) + (int64_t)Gy * (z + h));
+        const int64_t src = (int64_t)(sx + h) + Gx * ((int64_t)(sy + h) + (int64_t)Gy * (sz + h));
+        f[dst * C + c] = f[src * C + c];
     }
 }
 
@@ -324,17 +326,24 @@ int lf2mi_zero(void *ptr, int64_t count, int dtype, void *stream) {
 }
 
 int lf2mi_ghost_fill(void *field, int N, int halo, int C, int dtype, void *stream) {
-    if (int rc = check_field(N, halo, C, dtype)) return rc;
+    return lf2mi_ghost_fill_box(field, N, N, N, halo, C, dtype, stream);
+}
+
+int lf2mi_ghost_fill_box(void *field, int nx, int ny, int nz, int halo, int C, int dtype, void *stream) {
+    const int nmin = nx < ny ? (nx < nz ? nx : nz) : (ny < nz ? ny : nz);
+    if (nmin < 1 || halo < 0 || halo > nmin || C < 1)
+        return fail(LF2MI_EINVAL, "bad field geometry %dx%dx%d halo=%d C=%d", nx, ny, nz, halo, C);
+    if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
     if (!field) return fail(LF2MI_EINVAL, "lf2mi_ghost_fill: null field");
     if (halo == 0) return LF2MI_OK;
-    const int64_t G = N + 2 * halo;
-    const int64_t total = (G * G * G - (int64_t)N * N * N) * C;
+    const int64_t gross = (int64_t)(nx + 2 * halo) * (ny + 2 * halo) * (nz + 2 * halo);
+    const int64_t total = (gross - (int64_t)nx * ny * nz) * C;
     const unsigned g = grid_cap(total, 256);
     LF2MI_PROF("ghost_fill", as_stream(stream));
     if (dtype == LF2MI_F32)
-        ghost_fill_kernel<float><<<g, 256, 0, as_stream(stream)>>>((float *)field, N, halo, C);
+        ghost_fill_kernel<float><<<g, 256, 0, as_stream(stream)>>>((float *)field, nx, ny, nz, halo, C);
     else
-        ghost_fill_kernel<double><<<g, 256, 0, as_stream(stream)>>>((double *)field, N, halo, C);
+        ghost_fill_kernel<double><<<g, 256, 0, as_stream(stream)>>>((double *)field, nx, ny, nz, halo, C);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }

@@ -1871,6 +1871,90 @@ __global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// ---------------------------------------------------------------- particle statistics
+// The output[] reductions of updateVel / moveParticles (particles/LATfield2_Particles.hpp:
+// 788-807, 846-885) in FP64.  Each workgroup folds a grid-stride share of the particles and
+// then its lanes by a fixed tree; one thread per quantity folds the workgroups' partials in
+// index order.  The summation order depends only on np, so results repeat exactly.
+static constexpr int RED_Q = 8;        // quantities per call
+static constexpr int RED_WG = 256;
+static constexpr int RED_BLOCKS = 1024;
+enum { RED_NONE = -1, RED_OP_SUM = 0, RED_OP_MIN = 1, RED_OP_MAX = 2 };
+
+struct RedArgs {
+    int64_t np;
+    const float *vel32[3];
+    const float *mass32;   // per-particle masses (F32 container) or null
+    const double *vel64[3];
+    const double *mass64;  // per-particle masses (F64 container) or null
+    double mass;           // the global mass, when no per-particle one is used
+    int nq;
+    int q[RED_Q];
+    int op[RED_Q];
+};
+
+__device__ __forceinline__ double red_fold(int op, double a, double b) {
+    return op == RED_OP_SUM ? a + b : (op == RED_OP_MIN ? fmin(a, b) : fmax(a, b));
+}
+__device__ __forceinline__ double red_identity(int op) {
+    return op == RED_OP_SUM ? 0.0 : (op == RED_OP_MIN ? __builtin_inf() : -__builtin_inf());
+}
+
+template <bool F64>
+__global__ __launch_bounds__(RED_WG) void parts_reduce_kernel(RedArgs A, double *__restrict__ partial) {
+    double acc[RED_Q];
+#pragma unroll
+    for (int k = 0; k < RED_Q; ++k) acc[k] = red_identity(A.op[k]);
+    const int64_t stride = (int64_t)gridDim.x * RED_WG;
+    for (int64_t i = (int64_t)blockIdx.x * RED_WG + threadIdx.x; i < A.np; i += stride) {
+        double v[3], m;
+        if (F64) {
+            for (int d = 0; d < 3; ++d) v[d] = A.vel64[d][i];
+            m = A.mass64 ? A.mass64[i] : A.mass;
+        } else {
+            for (int d = 0; d < 3; ++d) v[d] = (double)A.vel32[d][i];
+            m = A.mass32 ? (double)A.mass32[i] : A.mass;
+        }
+        const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+#pragma unroll
+        for (int k = 0; k < RED_Q; ++k) {
+            if (k >= A.nq || A.op[k] == RED_NONE) continue;
+            double x;
+            switch (A.q[k]) {
+                case LF2MI_PQ_V2: x = v2; break;
+                case LF2MI_PQ_MVX: x = m * v[0]; break;
+                case LF2MI_PQ_MVY: x = m * v[1]; break;
+                case LF2MI_PQ_MVZ: x = m * v[2]; break;
+                case LF2MI_PQ_MASS: x = m; break;
+                default: x = 0.5 * m * v2; break;
+            }
+            acc[k] = red_fold(A.op[k], acc[k], x);
+        }
+    }
+    __shared__ double lds[RED_WG];
+    for (int k = 0; k < A.nq; ++k) {
+        lds[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int h = RED_WG / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) lds[threadIdx.x] = red_fold(A.op[k], lds[threadIdx.x], lds[threadIdx.x + h]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * RED_Q + k] = lds[0];
+        __syncthreads();
+    }
+}
+
+// out[k] = the reference's starting value folded with every workgroup's partial in order
+__global__ void parts_reduce_final_kernel(RedArgs A, const double *__restrict__ partial, int nblocks,
+                                          double *__restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= A.nq || A.op[k] == RED_NONE) return;
+    const int op = A.op[k];
+    double r = op == RED_OP_SUM ? 0.0 : (op == RED_OP_MIN ? 9223372036854775807.0 : -9223372036854775807.0);
+    for (int b = 0; b < nblocks; ++b) r = red_fold(op, r, partial[(int64_t)b * RED_Q + k]);
+    out[k] = r;
+}
+
 }  // namespace lf2mi
 
 // ---------------------------------------------------------------- container
@@ -1890,6 +1974,7 @@ struct lf2mi_parts_s {
     int64_t fx_n;
     unsigned int *vmax_f32;
     unsigned long long *vmax_f64;
+    double *red;   // statistics: RED_BLOCKS x RED_Q partials, then RED_Q results
     std::vector<void *> allocs;
     int64_t bytes;
     int plw;       // payload words (8 bytes) per particle, 0 without a payload
@@ -2147,6 +2232,7 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         if (flags & LF2MI_PARTS_ID) A((void **)&s.id, cap * sizeof(int64_t));
         A((void **)&p->vmax_f64, sizeof(unsigned long long));
     }
+    A((void **)&p->red, (size_t)(RED_BLOCKS + 1) * RED_Q * sizeof(double));
     if (rc) {
         for (void *q : p->allocs) hipFree(q);
         delete p;
@@ -2749,15 +2835,16 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
         LF2MI_LAUNCHED();
         A.fx = p->fx;
         FxScales S{};
-        if (kind == LF2MI_PROJ_SCALAR) {
-            // dimensionless weights (times the per-particle mass ratio) in 2^-32 units
+        if (kind == LF2MI_PROJ_SCALAR && !pm) {
+            // one global mass: dimensionless weights in 2^-32 units
             const double mref = A.mass_nv;
             A.fxs[0] = 4294967296.0 / mref;
             S.inv[0] = mref / 4294967296.0;
         } else {
-            // per component a power-of-two scale 2^(62 - e) with np max|q| max|v_a| max|v_b| < 2^e:
-            // no node (nor any partial sum) can leave the int64 range, the quantum is 2^-62 of
-            // that bound, and the integer sums do not depend on the order of the particles
+            // per component a power-of-two scale 2^(62 - e) with np max|q| max|v_a| max|v_b| < 2^e
+            // (the scalar kind with per-particle masses: np max|q| < 2^e): no node (nor any
+            // partial sum) can leave the int64 range, the quantum is 2^-62 of that bound, and
+            // the integer sums do not depend on the order of the particles
             unsigned int *dmax = nullptr, hmax[4] = {0u, 0u, 0u, 0u};
             LF2MI_HIP(hipMallocAsync((void **)&dmax, 4 * sizeof(unsigned int), st));
             fill_int_kernel<<<1, 64, 0, st>>>(reinterpret_cast<int *>(dmax), 4, 0);
@@ -2772,7 +2859,8 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
             static const int va[5][6] = {{-1}, {0, 1, 2}, {0, 0, 0, 1, 1, 2}, {0, 1, 2}, {0, 0, 0, 1, 1, 2}};
             static const int vb[5][6] = {{-1}, {-1, -1, -1}, {0, 1, 2, 1, 2, 2}, {-1, -1, -1}, {0, 1, 2, 1, 2, 2}};
             for (int c = 0; c < C; ++c) {
-                double bound = (double)p->np * (double)mx[0] * (double)mx[1 + va[kind][c]];
+                double bound = (double)p->np * (double)mx[0];
+                if (va[kind][c] >= 0) bound *= (double)mx[1 + va[kind][c]];
                 if (vb[kind][c] >= 0) bound *= (double)mx[1 + vb[kind][c]];
                 double sc = 1.0;
                 if (bound > 0.0 && bound < 1e300) {
@@ -2905,6 +2993,53 @@ int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_
         memcpy(&v, &bits, sizeof(v));
         *vmax2 = v;
     }
+    return LF2MI_OK;
+}
+
+int lf2mi_parts_reduce(lf2mi_parts p, int n, const int *quantity, const int *reduce_type, double *out, void *stream) {
+    if (!p || n < 0 || n > RED_Q || (n > 0 && (!quantity || !reduce_type || !out)))
+        return fail(LF2MI_EINVAL, "lf2mi_parts_reduce: bad argument (at most %d quantities)", RED_Q);
+    if (n == 0) return LF2MI_OK;
+    RedArgs A{};
+    A.np = p->np;
+    A.nq = n;
+    for (int k = 0; k < RED_Q; ++k) {
+        A.q[k] = 0;
+        A.op[k] = RED_NONE;
+    }
+    for (int k = 0; k < n; ++k) {
+        if (quantity[k] < LF2MI_PQ_V2 || quantity[k] > LF2MI_PQ_EKIN)
+            return fail(LF2MI_EINVAL, "lf2mi_parts_reduce: unknown quantity %d", quantity[k]);
+        const int rt = reduce_type[k];
+        A.q[k] = quantity[k];
+        A.op[k] = (rt & (LF2MI_RED_SUM | LF2MI_RED_SUM_LOCAL))   ? RED_OP_SUM
+                  : (rt & (LF2MI_RED_MIN | LF2MI_RED_MIN_LOCAL)) ? RED_OP_MIN
+                  : (rt & (LF2MI_RED_MAX | LF2MI_RED_MAX_LOCAL)) ? RED_OP_MAX
+                                                                 : RED_NONE;
+    }
+    A.mass = p->mass_global;
+    hipStream_t st = as_stream(stream);
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (p->np + RED_WG - 1) / RED_WG));
+    double *res = p->red + (int64_t)RED_BLOCKS * RED_Q;
+    LF2MI_PROF("parts_reduce", st);
+    if (p->dtype == LF2MI_F32) {
+        const Soa32 &s = p->b32[p->cur];
+        for (int d = 0; d < 3; ++d) A.vel32[d] = s.vel[d];
+        A.mass32 = p->mass_src ? s.mass : nullptr;
+        parts_reduce_kernel<false><<<nb, RED_WG, 0, st>>>(A, p->red);
+    } else {
+        for (int d = 0; d < 3; ++d) A.vel64[d] = p->b64.vel[d];
+        A.mass64 = p->mass_src ? p->b64.mass : nullptr;
+        parts_reduce_kernel<true><<<nb, RED_WG, 0, st>>>(A, p->red);
+    }
+    LF2MI_LAUNCHED();
+    parts_reduce_final_kernel<<<1, RED_Q, 0, st>>>(A, p->red, nb, res);
+    LF2MI_LAUNCHED();
+    double host[RED_Q];
+    LF2MI_HIP(hipMemcpyAsync(host, res, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    LF2MI_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < n; ++k)
+        if (A.op[k] != RED_NONE) out[k] = host[k];
     return LF2MI_OK;
 }
 

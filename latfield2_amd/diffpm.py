@@ -44,6 +44,8 @@ class DiffPM:
         call("lf2mi_diffpm_create", self.N, self.np, self.nsteps, self.mass, self.dt, self.code, ctypes.byref(h))
         self.h = h
         self._io = None
+        # counts the forwards: the device holds the checkpoints of the last one only
+        self.generation = 0
 
     def __del__(self):
         try:
@@ -88,13 +90,19 @@ class DiffPM:
         call("lf2mi_diffpm_forward", self.h, _ptr(pos0), _ptr(vel0), _ptr(target), io, _ptr(rho),
              ctypes.byref(loss), _stream())
         self._io = pos0.dtype
+        self.generation += 1
         return (loss.value, rho) if want_rho else loss.value
 
-    def backward(self, want_vel: bool = True):
-        """dL/dpos0 and dL/dvel0 of the last forward, shaped (np, 3)"""
+    def backward(self, want_vel: bool = True, generation: int | None = None):
+        """dL/dpos0 and dL/dvel0 of the last forward, shaped (np, 3); with `generation`,
+        raise unless the last forward is that one"""
         torch = _torch()
         if self._io is None:
             raise RuntimeError("backward before forward")
+        if generation is not None and generation != self.generation:
+            raise RuntimeError(f"the model's checkpoints belong to forward {self.generation}, not to forward "
+                               f"{generation}: run each forward's backward before the next forward on the same "
+                               "DiffPM, or use one DiffPM per evaluation")
         gp = torch.empty((self.np, 3), dtype=self._io, device="cuda")
         gv = torch.empty((self.np, 3), dtype=self._io, device="cuda") if want_vel else None
         call("lf2mi_diffpm_backward", self.h, _ptr(gp), _ptr(gv), _stream())
@@ -115,7 +123,10 @@ def _autograd_function():
 
     class PMDensityLoss(torch.autograd.Function):
         """L = sum_cells (rho_final - target)^2 through the GPU leapfrog; the backward
-        runs the hand-written adjoint kernels (not torch autograd)."""
+        runs the hand-written adjoint kernels (not torch autograd).  The model keeps the
+        checkpoints of its last forward only, so a backward whose forward was followed by
+        another forward of the same model raises instead of differentiating the wrong
+        trajectory."""
 
         @staticmethod
         def forward(ctx, pos0, vel0, target, model):
@@ -123,6 +134,7 @@ def _autograd_function():
                                       None if vel0 is None else vel0.detach().contiguous(),
                                       None if target is None else target.detach().contiguous(), want_rho=True)
             ctx.model = model
+            ctx.generation = model.generation
             ctx.pos_shape = pos0.shape
             ctx.has_vel = vel0 is not None
             ctx.has_target = target is not None
@@ -132,7 +144,7 @@ def _autograd_function():
         @staticmethod
         def backward(ctx, g):
             rho, target = ctx.saved_tensors
-            gp, gv = ctx.model.backward(want_vel=ctx.has_vel)
+            gp, gv = ctx.model.backward(want_vel=ctx.has_vel, generation=ctx.generation)
             s = g.to(gp.dtype)
             gt = (-2.0 * (rho - target.reshape(rho.shape)) * s).reshape(target.shape) if ctx.has_target else None
             return (gp * s).reshape(ctx.pos_shape), (gv * s).reshape(ctx.pos_shape) if gv is not None else None, gt, None

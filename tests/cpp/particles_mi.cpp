@@ -116,11 +116,33 @@ int main(int argc, char **argv) {
     for (x.first(); x.test(); x.next()) phi(x) = 3.0;
     phi.toDevice();
     Field<Real> *fields[1] = {&phi};
-    const Real vmax = parts.updateVel(gpu_gather_kick(), 0.01, fields, 1);
-    std::printf("updateVel: max |v| = %.9f (expected sqrt 3)\n", (double)vmax);
+    // the reference's updateVel signature with outputs (particles/LATfield2_Particles.hpp:217-224):
+    // slots |v|^2, m v_x, m v_y, m v_z, m, m |v|^2 / 2 of the device functors
+    double stats[6] = {-1, -1, -1, -1, -1, 42.0};
+    int rtype[6] = {SUM, MAX_LOCAL, SUM, MIN, SUM_LOCAL, 0};
+    const Real vmax = parts.updateVel(updateVel_cicKick, 0.01, fields, 1, NULL, stats, rtype, 6);
+    const double np3 = (double)N * N * N;
+    std::printf("updateVel: max |v| = %.9f (expected sqrt 3); sum |v|^2 %.9g, max m v_x %.9g, sum m v_y %.9g, "
+                "min m v_z %.9g, sum m %.9g, untouched %.3g\n",
+                (double)vmax, stats[0], stats[1], stats[2], stats[3], stats[4], stats[5]);
     if (std::fabs(vmax - std::sqrt(3.0)) > 10 * tol) ++fails;
+    if (std::fabs(stats[0] / (3.0 * np3) - 1) > 10 * tol || std::fabs(stats[1] / 0.1 - 1) > 10 * tol ||
+        std::fabs(stats[2] / (0.1 * np3) - 1) > 10 * tol || std::fabs(stats[3] / 0.1 - 1) > 10 * tol ||
+        std::fabs(stats[4] / (0.1 * np3) - 1) > 10 * tol || stats[5] != 42.0)
+        ++fails;
+    // the identity functor: velocities unchanged, the same outputs
+    double out2[2] = {0, 0};
+    int rtype2[2] = {MIN, SUM};
+    const Real vmax2 = parts.updateVel(updateVel_simple, 0.01, NULL, 0, NULL, out2, rtype2, 2);
+    std::printf("updateVel_simple: max |v| = %.9f, min |v|^2 %.9g, sum m v_x %.9g\n", (double)vmax2, out2[0], out2[1]);
+    if (std::fabs(vmax2 - std::sqrt(3.0)) > 10 * tol || std::fabs(out2[0] - 3.0) > 10 * tol ||
+        std::fabs(out2[1] / (0.1 * np3) - 1) > 10 * tol)
+        ++fails;
     const double dt = 0.25 / N;  // a quarter cell
-    parts.moveParticles(gpu_drift(), dt);
+    double mout[1] = {0};
+    int mtype[1] = {SUM};
+    parts.moveParticles(move_particles_simple, dt, NULL, 0, NULL, mout, mtype, 1);
+    if (std::fabs(mout[0] / (3.0 * np3) - 1) > 10 * tol) ++fails;
     std::vector<part_simple> out;
     parts.getParticles(out);
     double e_pos = 0.0;
@@ -201,7 +223,7 @@ int main(int argc, char **argv) {
         std::printf("struct vectors: default (vel, individual mass) and oset = {info mass, mom}: max rel error %.2e\n",
                     b_err);
         if (b_err > stol) ++fails;
-        wp.moveParticles(gpu_drift(), 3.0 / N);  // 1.5 cells: many cross a brick face
+        wp.moveParticles(move_particles_simple, 3.0 / N);  // 1.5 cells: many cross a brick face
         std::vector<part_weighted> back;
         wp.getParticles(back);
         long bad = (long)back.size() != k ? 1 : 0;

@@ -29,3 +29,21 @@ def test_shim_host_geometry(tmp_path):
 def test_dropin_programs_compile(prog, defs):
     subprocess.run([CXX, "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", *defs, "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", prog)], check=True)
+
+
+REFERENCE = "/root/reference"
+REFERENCE_DRIVERS = ["examples/poissonSolver.cpp", "benchmarks_tests/wave_test.cpp",
+                     "benchmarks_tests/wave_test_back.cpp", "examples/fft_c2c_r2c_sametime.cpp",
+                     "examples/fft.cpp", "examples/gettingStarted.cpp", "examples/test_sum.cpp"]
+
+
+@pytest.mark.skipif(CXX is None, reason="no host C++ compiler")
+@pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="the reference tree is not present")
+@pytest.mark.parametrize("driver", REFERENCE_DRIVERS)
+@pytest.mark.parametrize("defs", [[], ["-DSINGLE"]])
+def test_reference_drivers_compile_against_the_dropin_header(driver, defs):
+    """the reference's own programs, read where they lie, compile unmodified against
+    include/LATfield2.hpp (parallel, COUT, Lattice, Site/rKSite/cKSite, Field, PlanFFT,
+    MPI_Wtime as they use them)"""
+    subprocess.run([CXX, "-fsyntax-only", "-std=c++17", *defs, "-I", os.path.join(ROOT, "include"),
+                    os.path.join(REFERENCE, driver)], check=True)

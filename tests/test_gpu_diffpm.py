@@ -194,6 +194,32 @@ def test_autograd_harness():
     assert rel_l2(target.grad.cpu().numpy(), gt.cpu().numpy()) <= 1e-6
 
 
+def test_autograd_two_evaluations_of_one_model_refuse_the_stale_backward():
+    """L(a) + L(b) on one DiffPM: the device holds the checkpoints of b only, so the first
+    node's backward must raise rather than return b's gradients for a"""
+    lf, DiffPM, pm_density_loss = _lib()
+    N, nsteps, dt = 16, 1, 0.02
+    np_ = N ** 3
+    announce("diffpm two evaluations before one backward")
+    m = DiffPM(N, np_, nsteps, 1.0 / np_, dt, lf.F32)
+    a = dev(lattice_positions(N, N, 0.5, seed=6), torch.float32).requires_grad_()
+    b = dev(lattice_positions(N, N, 0.5, seed=7), torch.float32).requires_grad_()
+    loss = pm_density_loss(a, None, None, m) + pm_density_loss(b, None, None, m)
+    with pytest.raises(RuntimeError, match="checkpoints belong to forward"):
+        loss.backward()
+    # one model per evaluation works
+    m2 = DiffPM(N, np_, nsteps, 1.0 / np_, dt, lf.F32)
+    a.grad = None
+    b.grad = None
+    loss = pm_density_loss(a, None, None, m) + pm_density_loss(b, None, None, m2)
+    loss.backward()
+    ma = DiffPM(N, np_, nsteps, 1.0 / np_, dt, lf.F32)
+    ma.forward(a.detach())
+    ga, _ = ma.backward(want_vel=False)
+    torch.cuda.synchronize()
+    assert rel_l2(a.grad.cpu().numpy(), ga.cpu().numpy()) <= 1e-6
+
+
 def test_diffpm_errors():
     lf, DiffPM, _ = _lib()
     from latfield2_amd import _lib as raw
