@@ -644,7 +644,9 @@ class Field {
     // the host array (LATfield2_Field_decl.hpp:1984), brought up to date and marked as
     // possibly written
     FieldType *&data() {
-        hp();
+        // a field that aliases another's storage may see the host copy allocated through
+        // the other field, so data_ is refreshed here
+        data_ = hp();
         return data_;
     }
     // mark the host copy as written (after writes through data_ kept from earlier)

@@ -255,7 +255,7 @@ template <typename T> struct StridedArgs {
     int ncols;                  // valid columns per row
     const cplx<T> *tw;
     // Green fusion (MODE 1): element (col, e, batch) has kx = col / C, ky = e, kz = batch
-    const double *green_s;      // s[k] = N^2 (2 - 2 cos(2 pi k / N)), FP64 table
+    const double *green_s;      // s[k] = N^2 (2 - 2 cos(2 pi k / N)) = 4 N^2 sin^2(pi k / N), FP64 table
     const float *green_sf;      // the same table rounded to FP32 (FP32 path)
     double green_scale;
     int C;
@@ -912,9 +912,13 @@ static int poisson(lf2mi_fft_plan_s *p, const void *rho, void *phi, double shift
 }
 
 // ---------------------------------------------------------------- Green kernel (drop-in)
+// lambda(k) = 2 (sum_i cos(2 pi k_i / N) - 3) / res^2 (examples/poissonSolver.cpp:100-104),
+// evaluated as -sum_i 4 sin^2(pi k_i / N) / res^2, the same operator without the cancellation
+// of the cosine sum against 3 (at N = 1024 and k = (1,0,0) that cancellation alone costs
+// 2e-11 relative in FP64); s4[q] = 4 sin^2(pi q / N)
 template <typename T>
 __global__ void green_kernel(cplx<T> *__restrict__ k, int N, int C, int kh, double inv_res2, double scale,
-                             const double *__restrict__ cosv) {
+                             const double *__restrict__ s4) {
     const int H = N / 2 + 1;
     const int64_t R = H + 2 * kh, KG = N + 2 * kh;
     const int64_t total = (int64_t)H * N * N;
@@ -926,7 +930,7 @@ __global__ void green_kernel(cplx<T> *__restrict__ k, int N, int C, int kh, doub
         const int64_t o = ((kx + kh) + R * ((kz + kh) + KG * (ky + kh))) * C;
         double f = 0.0;
         if (kx | ky | kz) {
-            const double lam = 2.0 * (cosv[kx] + cosv[ky] + cosv[kz] - 3.0) * inv_res2;
+            const double lam = -(s4[kx] + s4[ky] + s4[kz]) * inv_res2;
             f = scale / lam;
         }
         for (int c = 0; c < C; ++c) {
@@ -980,7 +984,9 @@ static int plan_create(int N, int C, int rhalo, int khalo, int dtype, int c2c, l
         twd[2 * q + 1] = sin(a);
         twf[2 * q] = (float)twd[2 * q];
         twf[2 * q + 1] = (float)twd[2 * q + 1];
-        gs[q] = (double)N * N * (2.0 - 2.0 * cos(a));
+        // 2 - 2 cos(a) evaluated as 4 sin^2(a / 2): no cancellation for small k
+        const double sh = sin(M_PI * (double)q / (double)N);
+        gs[q] = (double)N * N * 4.0 * sh * sh;
     }
     // workspace rows padded to a multiple of 16 complex values (128 B for float2)
     p->wcols = (((int64_t)(N / 2 + 1) * C + 15) / 16) * 16;
@@ -1069,14 +1075,17 @@ int lf2mi_fft_execute(lf2mi_fft_plan p, int direction, void *rfield, void *kfiel
 
 int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, double scale, int dtype, void *stream) {
     if (!kfield || N < 1 || C < 1 || khalo < 0 || !(res > 0)) return fail(LF2MI_EINVAL, "lf2mi_poisson_green: bad argument");
-    double *cosv = nullptr;
+    double *s4v = nullptr;
     double *host = (double *)malloc(sizeof(double) * N);
     if (!host) return fail(LF2MI_ENOMEM, "host allocation failed");
-    for (int q = 0; q < N; ++q) host[q] = cos(2.0 * M_PI * q / N);
+    for (int q = 0; q < N; ++q) {
+        const double sh = sin(M_PI * (double)q / (double)N);
+        host[q] = 4.0 * sh * sh;
+    }
     hipStream_t st = as_stream(stream);
-    hipError_t e = hipMallocAsync((void **)&cosv, sizeof(double) * N, st);
+    hipError_t e = hipMallocAsync((void **)&s4v, sizeof(double) * N, st);
     if (e != hipSuccess) { free(host); return hip_fail(e, "hipMallocAsync", __FILE__, __LINE__); }
-    e = hipMemcpyAsync(cosv, host, sizeof(double) * N, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(s4v, host, sizeof(double) * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     free(host);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
@@ -1084,15 +1093,15 @@ int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, doubl
     const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
     LF2MI_PROF("poisson_green", st);
     if (dtype == LF2MI_F32)
-        green_kernel<float><<<g, 256, 0, st>>>((float2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, cosv);
+        green_kernel<float><<<g, 256, 0, st>>>((float2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, s4v);
     else if (dtype == LF2MI_F64)
-        green_kernel<double><<<g, 256, 0, st>>>((double2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, cosv);
+        green_kernel<double><<<g, 256, 0, st>>>((double2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, s4v);
     else {
-        hipFreeAsync(cosv, st);
+        hipFreeAsync(s4v, st);
         return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
     }
     LF2MI_LAUNCHED();
-    LF2MI_HIP(hipFreeAsync(cosv, st));
+    LF2MI_HIP(hipFreeAsync(s4v, st));
     return LF2MI_OK;
 }
 

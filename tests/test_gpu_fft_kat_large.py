@@ -105,7 +105,8 @@ def test_poisson_closed_form_at_benchmark_sizes(N, prec, tol):
         ph = ((a * x[None, None, :] + b * x[None, :, None] + c * x[:, None, None]) % N).to(torch.float64)
         want = torch.cos(2.0 * math.pi * ph / N)
         del ph
-        K2 = sum(N * N * (2.0 - 2.0 * math.cos(2.0 * math.pi * q / N)) for q in (a, b, c))
+        # 2 - 2 cos(x) = 4 sin^2(x / 2), the form without cancellation at small k
+        K2 = sum(N * N * 4.0 * math.sin(math.pi * q / N) ** 2 for q in (a, b, c))
         rho.interior()[..., 0] = want.to(TD[prec])
         want.mul_(-1.0 / K2)
         wmax = float(want.abs().max())
