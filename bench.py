@@ -10,7 +10,7 @@ the device and resident in HBM before the timed region.
 Also measured (outside the timed region): the plain FP64 GPU path on the same
 workload, the FP32-vs-FP64 potential difference, per-kernel times with HIP events on
 the launch stream (roofline of the dominant kernel), and the FP64 CPU oracle on a
-bounded sample (cpu_baseline).
+full workload, on all the host threads granted and on one (cpu_baseline).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-fp64] [--no-cpu]
 """
@@ -96,37 +96,51 @@ def timed_steps(torch, lf, parts, plan, rho, phi, steps, warmup, dt, on_timed_st
     return ev[0].elapsed_time(ev[steps]) / steps, wall / steps, median
 
 
-def cpu_baseline(lf, torch, threads):
-    """FP64 oracle PM step on a 256^3 sample of the same workload, scaled x8 to 512^3."""
+def host_cores() -> tuple[int, int]:
+    """(cores this process may run on, threads the oracle uses): the affinity mask, bounded by
+    OMP_NUM_THREADS where the host sets a per-job share"""
+    cores = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(cores, int(share)) if share.isdigit() and int(share) > 0 else cores
+    return cores, threads
+
+
+def cpu_baseline(lf, torch, N, threads, cores):
+    """FP64 oracle PM step on the full workload (N^3 mesh, N^3 particles of the same blob IC),
+    timed on `threads` host threads and on one."""
     import numpy as np
 
     import oracle as orc
-    Ns, Nps = 256, 256 ** 3
-    parts = lf.Particles(Ns, Nps, lf.F64, mass=1.0 / Nps)
-    parts.ic_blobs(Nps, nblobs=4096, alpha=1.5, sigma_cells=2.0, frac_uniform=0.2, sigma_v=0.01, seed=3)
+    Np = N ** 3
+    parts = lf.Particles(N, Np, lf.F64, mass=1.0 / Np)
+    parts.ic_blobs(Np, nblobs=4096, alpha=1.5, sigma_cells=2.0, frac_uniform=0.2, sigma_v=0.01, seed=3)
     d = parts.download()
     del parts
     torch.cuda.empty_cache()
-    orc.set_threads(threads)
     pos = np.ascontiguousarray(d["pos"])
     vel = np.ascontiguousarray(d["vel"])
+    del d
     pos1, vel1 = pos.copy(), vel.copy()  # pm_step advances its inputs in place
+    orc.set_threads(threads)
     t0 = time.perf_counter()
-    out = orc.pm_step(pos, vel, Ns, 0.01, 0.01, mass_global=1.0 / Nps, want_acc=False)
+    out = orc.pm_step(pos, vel, N, 0.01, 0.01, mass_global=1.0 / Np, want_acc=False)
     sec = time.perf_counter() - t0
-    # the same sample on one core
+    stages = {k: round(v, 1) for k, v in out["stage_ms"].items()}
+    del out, pos, vel
+    log(f"CPU oracle step {sec:.2f} s on {threads} threads; now on one thread")
     orc.set_threads(1)
     t0 = time.perf_counter()
-    orc.pm_step(pos1, vel1, Ns, 0.01, 0.01, mass_global=1.0 / Nps, want_acc=False)
+    orc.pm_step(pos1, vel1, N, 0.01, 0.01, mass_global=1.0 / Np, want_acc=False)
     sec1 = time.perf_counter() - t0
     orc.set_threads(threads)
-    return {"value": round(sec * 1e3 * 8.0, 1), "unit": "ms per PM step at 512^3 (scaled)", "cores": threads,
+    return {"value": round(sec * 1e3, 1), "unit": f"ms per PM step at {N}^3", "cores": threads,
             "kind": "port",
-            "sample": f"one FP64 oracle PM step on a 256^3 mesh with 256^3 particles of the same blob IC "
-                      f"({sec:.2f} s), scaled x8 to 512^3 by mesh sites and particle count",
-            "stage_ms_256": {k: round(v, 1) for k, v in out["stage_ms"].items()},
-            "value_1core": round(sec1 * 1e3 * 8.0, 1),
-            "sample_1core": f"the same 256^3 step on one thread ({sec1:.2f} s), scaled x8"}
+            "sample": f"one whole FP64 oracle PM step (deposit, fold, FFT Poisson solve, gather + kick + "
+                      f"drift) on the {N}^3 mesh with {N}^3 particles of the same blob IC, {threads} OpenMP "
+                      f"threads ({cores} cores in this process's affinity mask)",
+            "stage_ms": stages,
+            "value_1core": round(sec1 * 1e3, 1),
+            "sample_1core": f"the same {N}^3 step on one thread"}
 
 
 def main():
@@ -251,10 +265,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        log(f"CPU oracle sample with {threads} threads")
-        cpu = cpu_baseline(lf, torch, threads)
-        log(f"CPU baseline {cpu['value']} ms (scaled)")
+        cores, threads = host_cores()
+        log(f"CPU oracle at {N}^3 with {threads} threads ({cores} cores in the affinity mask)")
+        cpu = cpu_baseline(lf, torch, N, threads, cores)
+        log(f"CPU baseline {cpu['value']} ms, one thread {cpu['value_1core']} ms")
 
     step_bytes = sum((algorithmic_bytes(k, N, Np, 4, np_sparse) or 0) * v["launches_per_step"]
                      for k, v in kernels.items() if k != "rebucket_scatter" and k != "zero")

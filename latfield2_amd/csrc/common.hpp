@@ -24,6 +24,9 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// clears the ghost sites of a cubic field (field.hip)
+int zero_ghosts(void *field, int N, int h, int C, int dtype, hipStream_t st);
+
 template <typename T> struct vec2;
 template <> struct vec2<float> { using type = float2; };
 template <> struct vec2<double> { using type = double2; };

@@ -99,7 +99,8 @@ __global__ void zero_kernel(T *__restrict__ p, int64_t n) {
 // once: z-slabs (all x, y), then y-slabs (z interior, all x), then x-slabs (y, z
 // interior).  Each copies from its periodic image, so the result does not depend on
 // the order (LATfield2_Field_decl.hpp:905-926 computes the same source index).
-template <typename T>
+// ZERO: the ghost sites are cleared instead (a field whose interior is then overwritten).
+template <typename T, bool ZERO = false>
 __global__ void ghost_fill_kernel(T *__restrict__ f, int nx, int ny, int nz, int h, int C) {
     const int Gx = nx + 2 * h, Gy = ny + 2 * h;
     const int64_t cz = 2LL * h * Gx * Gy;        // z-slabs
@@ -138,7 +139,7 @@ __global__ void ghost_fill_kernel(T *__restrict__ f, int nx, int ny, int nz, int
         const int sz = z < 0 ? z + nz : (z >= nz ? z - nz : z);
         const int64_t dst = (int64_t)(x + h) + Gx * ((int64_t)(y + h) + (int64_t)Gy * (z + h));
         const int64_t src = (int64_t)(sx + h) + Gx * ((int64_t)(sy + h) + (int64_t)Gy * (sz + h));
-        f[dst * C + c] = f[src * C + c];
+        f[dst * C + c] = ZERO ? T(0) : f[src * C + c];
     }
 }
 
@@ -237,6 +238,19 @@ static int check_field(int N, int halo, int C, int dtype) {
     return LF2MI_OK;
 }
 
+// clears the ghost sites of a cubic field of ghost width h (the interior is left alone)
+int zero_ghosts(void *field, int N, int h, int C, int dtype, hipStream_t st) {
+    if (h == 0) return LF2MI_OK;
+    const int64_t G = N + 2 * h;
+    const int64_t total = (G * G * G - (int64_t)N * N * N) * C;
+    const unsigned g = grid_cap(total, 256);
+    if (dtype == LF2MI_F32)
+        ghost_fill_kernel<float, true><<<g, 256, 0, st>>>((float *)field, N, N, N, h, C);
+    else
+        ghost_fill_kernel<double, true><<<g, 256, 0, st>>>((double *)field, N, N, N, h, C);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
 }  // namespace lf2mi
 
 using namespace lf2mi;

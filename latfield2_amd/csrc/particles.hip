@@ -1141,6 +1141,323 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
     }
 }
 
+// ---------------------------------------------------------------- scalar deposit, periodic target
+// Four kernels replace the item kernels when the target is periodic, the brick edge 8, the
+// number of bricks per row a multiple of TILE_B and the mass global (the PM step's deposit).
+// A brick's nodes are [0,8)^3 relative to its origin; its particles also reach the first
+// node layer of the bricks above it along x, y and z.
+//  * deposit_tiles_kernel: one workgroup per tile of TILE_B bricks along x, one wave per
+//    brick.  A brick with fewer than `dense_min` particles sums their terms in the wave's LDS
+//    window of 9^3 nodes (a denser brick leaves it empty).  Within the tile each brick's
+//    x = 8 plane is added into the next brick's x = 0 plane on chip; the tile's own nodes are
+//    then written once, TILE_X consecutive x per row (stored, or added to the field).  The layer
+//    that belongs to the bricks above in y or z goes to two shell buffers and the tile's last
+//    x = 8 plane to a seam buffer, one value per node.  No global atomics, no sort.
+//  * dense_groups_kernel + deposit_dense_kernel: the bricks with dense_min or more particles,
+//    in groups of up to DEP_MERGE items (a work list shared out statically among persistent
+//    workgroups); each particle's 8 terms go straight into a 9^3 LDS window (no sort: a
+//    wave's 64 particles of a crowded brick rarely share a cell), added to the field with one
+//    global atomic per non-zero node after the tiles kernel's stores.
+//  * deposit_fold_kernel: every node in the lowest y or z layer of a brick, or in the first
+//    x plane of a tile, adds the shell and seam values meant for it (plain read-modify-write,
+//    one thread per node, x fastest).
+// Window sums are exact 64-bit fixed point: per term a power-of-two scale 2^(51 - e) of a
+// bound n q < 2^e (ACC float: n = dense_min for the tiles, whose x = 0 planes then take at
+// most twice that, the group's count for the dense kernel), or the term quantised on its own
+// (ACC fixed point, order independent).
+static constexpr int TILE_B = 8;           // bricks per tile (one wave each)
+static constexpr int TILE_X = 8 * TILE_B;  // nodes per tile row
+
+template <typename ACC>
+__device__ __forceinline__ unsigned long long dep_term(float q, float w, double sc, double fxs) {
+    if constexpr (sizeof(ACC) == 8) return AccOps<ACC>::term(q, w, fxs);
+    else return to_window(q * w, sc);
+}
+
+// scale of a window whose bound is n particles of value q (float accumulation)
+__device__ __forceinline__ double bound_scale(int n, float q) {
+    const double bound = (double)n * (double)fabsf(q);
+    if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
+    int e;
+    frexp(bound, &e);
+    return ldexp(1.0, 51 - e);
+}
+
+// Shell and seam buffers (row = by + NB bz, X global, tile = X / TILE_X):
+//   ysh[row][z in 0..8][X]: nodes with y = 8 of the row's bricks (they belong to row by + 1);
+//   zsh[row][y in 0..7][X]: nodes with z = 8, y < 8 (row bz + 1);
+//   xsh[row][tile][y + 9 z]: the x = 8 plane of the tile's last brick, addressed by the tile
+//   whose first plane it is (the next one, periodically).
+template <typename ACC>
+__global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, int dense_min, int set,
+                                                                    ACC *__restrict__ ysh, ACC *__restrict__ zsh,
+                                                                    ACC *__restrict__ xsh) {
+    constexpr int B = 8, NWE = 9, NW3 = NWE * NWE * NWE, PL = NWE * NWE, SP = 3;
+    constexpr bool FX = sizeof(ACC) == 8;
+    __shared__ unsigned long long win[TILE_B][NW3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int NB = A.g.NB, N = A.g.N;
+    const int ntile = NB / TILE_B;
+    const int tile = blockIdx.x % ntile, row = blockIdx.x / ntile;
+    const int by = row % NB, bz = row / NB;
+    const int64_t brick = (int64_t)row * NB + tile * TILE_B + wave;  // bricks are numbered x fastest
+    unsigned long long *w = win[wave];
+    for (int i = lane; i < NW3; i += 64) w[i] = 0ull;
+    const float q = A.mass_nv;
+    const double sc = FX ? 1.0 : bound_scale(dense_min, q);
+    const int a = A.start[brick], n = A.start[brick + 1] - a;
+    if (n < dense_min) {
+        int c[SP];
+        float ux[SP], uy[SP], uz[SP];
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            const int j = lane + 64 * k;
+            c[k] = -1;
+            if (j < n) {
+                c[k] = A.s.cell[a + j];
+                ux[k] = A.s.off[0][a + j];
+                uy[k] = A.s.off[1][a + j];
+                uz[k] = A.s.off[2][a + j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SP; ++k) {
+            if (c[k] < 0) continue;
+            int x, y, z;
+            decode_cell(c[k], A.g, x, y, z);
+            const int base = (x & (B - 1)) + NWE * ((y & (B - 1)) + NWE * (z & (B - 1)));
+            const float wx[2] = {1.0f - ux[k], ux[k]}, wy[2] = {1.0f - uy[k], uy[k]}, wz[2] = {1.0f - uz[k], uz[k]};
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+                for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                    for (int cx = 0; cx < 2; ++cx)
+                        atomicAdd(&w[base + cx + NWE * (cy + NWE * cz)],
+                                  dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], sc, A.fxs[0]));
+        }
+    }
+    __syncthreads();
+    // the previous brick's x = 8 plane joins this brick's x = 0 plane
+    if (wave > 0)
+        for (int yz = lane; yz < PL; yz += 64) w[NWE * yz] += win[wave - 1][B + NWE * yz];
+    __syncthreads();
+    const double inv = 1.0 / sc;
+    auto cvt = [&](unsigned long long v) -> ACC {
+        if constexpr (FX) return v;
+        else return (float)((double)(long long)v * inv);
+    };
+    const int X0 = tile * TILE_X;
+    // own nodes, TILE_X consecutive x per row
+    for (int t = threadIdx.x; t < TILE_X * B * B; t += 64 * TILE_B) {
+        const int xi = t % TILE_X, yz = t / TILE_X, y = yz % B, z = yz / B;
+        const unsigned long long v = win[xi / B][(xi % B) + NWE * (y + NWE * z)];
+        const int64_t idx = site_index(N, A.h, X0 + xi, by * B + y, bz * B + z);
+        if constexpr (FX) {
+            if (set) A.fx[idx] = v;
+            else if (v) A.fx[idx] += v;
+        } else {
+            const float f = cvt(v);
+            if (set) A.field[idx] = f;
+            else if (v) A.field[idx] += f;
+        }
+    }
+    // y = 8 layer (z in 0..8), then z = 8 layer (y in 0..7)
+    for (int t = threadIdx.x; t < TILE_X * (NWE + B); t += 64 * TILE_B) {
+        const int xi = t % TILE_X, r = t / TILE_X;
+        const int y = r < NWE ? B : r - NWE, z = r < NWE ? r : B;
+        const ACC v = cvt(win[xi / B][(xi % B) + NWE * (y + NWE * z)]);
+        if (r < NWE) ysh[((int64_t)row * NWE + z) * N + X0 + xi] = v;
+        else zsh[((int64_t)row * B + y) * N + X0 + xi] = v;
+    }
+    // the tile's last x = 8 plane, for the next tile's first plane
+    const int nt = tile + 1 == ntile ? 0 : tile + 1;
+    for (int yz = threadIdx.x; yz < PL; yz += 64 * TILE_B)
+        xsh[((int64_t)row * ntile + nt) * PL + yz] = cvt(win[TILE_B - 1][B + NWE * yz]);
+}
+
+// one thread per node that receives shell or seam values, x fastest: the lowest y layer of
+// each brick (N x NB x N nodes), then the lowest z layer without it (N x 7 NB x NB), then the
+// first x plane of each tile without both (ntile x 7 NB x 7 NB, x slowest)
+template <typename ACC>
+__global__ __launch_bounds__(256) void deposit_fold_kernel(DepArgs A, const ACC *__restrict__ ysh,
+                                                           const ACC *__restrict__ zsh, const ACC *__restrict__ xsh) {
+    constexpr int B = 8, NWE = 9, PL = NWE * NWE;
+    constexpr bool FX = sizeof(ACC) == 8;
+    const unsigned N = A.g.N, NB = A.g.NB, ntile = NB / TILE_B;
+    const unsigned ny = N * NB * N, nz = N * (N - NB) * NB, nx = ntile * (N - NB) * (N - NB);
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ny + nz + nx) return;
+    // divisions by N and NB as shifts when N is a power of two (wave-uniform branch)
+    const int lg = A.g.lgN;
+    auto divN = [&](unsigned v) { return lg >= 0 ? v >> lg : v / N; };
+    auto divNB = [&](unsigned v) { return lg >= 0 ? v >> (lg - 3) : v / NB; };
+    unsigned X, Y, Z;
+    if (t < ny) {
+        const unsigned r = divN(t);
+        X = t - r * N;
+        Z = divNB(r);
+        Y = (r - Z * NB) * B;
+    } else if (t < ny + nz) {
+        // r = yy + 7 NB zb: yy = 7 yb + k, k in 0..6
+        const unsigned u = t - ny, r = divN(u);
+        X = u - r * N;
+        const unsigned zb = divNB(r) / (B - 1), yy = r - zb * (N - NB), yb = yy / (B - 1);
+        Y = yb * B + 1 + (yy - yb * (B - 1));
+        Z = zb * B;
+    } else {
+        const unsigned u = t - ny - nz, m = N - NB;
+        const unsigned r = divNB(u) / (B - 1), yy = u - r * m;  // r = zz + m tile
+        const unsigned tl = divNB(r) / (B - 1), zz = r - tl * m;
+        const unsigned yb = yy / (B - 1), zb = zz / (B - 1);
+        X = tl * TILE_X;
+        Y = yb * B + 1 + (yy - yb * (B - 1));
+        Z = zb * B + 1 + (zz - zb * (B - 1));
+    }
+    const unsigned by = Y / B, bz = Z / B, ly = Y % B, lz = Z % B;
+    const unsigned bym = by == 0 ? NB - 1 : by - 1, bzm = bz == 0 ? NB - 1 : bz - 1;
+    ACC v = ACC(0);
+    if (ly == 0) {
+        v += ysh[((int64_t)(bym + NB * bz) * NWE + lz) * N + X];
+        if (lz == 0) v += ysh[((int64_t)(bym + NB * bzm) * NWE + B) * N + X];
+    }
+    if (lz == 0) v += zsh[((int64_t)(by + NB * bzm) * B + ly) * N + X];
+    if (X % TILE_X == 0) {
+        const unsigned tl = X / TILE_X;
+        v += xsh[((int64_t)(by + NB * bz) * ntile + tl) * PL + ly + NWE * lz];
+        if (ly == 0) v += xsh[((int64_t)(bym + NB * bz) * ntile + tl) * PL + B + NWE * lz];
+        if (lz == 0) v += xsh[((int64_t)(by + NB * bzm) * ntile + tl) * PL + ly + NWE * B];
+        if (ly == 0 && lz == 0) v += xsh[((int64_t)(bym + NB * bzm) * ntile + tl) * PL + B + NWE * B];
+    }
+    if (v == ACC(0)) return;
+    const int64_t idx = site_index((int)N, A.h, (int)X, (int)Y, (int)Z);
+    if constexpr (FX) A.fx[idx] += v;
+    else A.field[idx] += v;
+}
+
+// work list of the dense kernel: (brick, group) pairs of the bricks with dense_min or more
+// particles, a group being up to DEP_MERGE * ITEM_P consecutive particles of the brick
+// one atomic per workgroup of DG_WG threads (a scan over the workgroup)
+static constexpr int DG_WG = 1024;
+__global__ __launch_bounds__(DG_WG) void dense_groups_kernel(const int *__restrict__ start, int64_t nbk,
+                                                             int dense_min, int2 *__restrict__ groups,
+                                                             int *__restrict__ ngroups) {
+    constexpr int GP = DEP_MERGE * ITEM_P;
+    __shared__ int wsum[DG_WG / 64];
+    __shared__ int wbase;
+    const int64_t b = (int64_t)blockIdx.x * DG_WG + threadIdx.x;
+    int g = 0;
+    if (b < nbk) {
+        const int n = start[b + 1] - start[b];
+        if (n >= dense_min) g = (n + GP - 1) / GP;
+    }
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int incl = g;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < DG_WG / 64; ++w) {
+            const int v = wsum[w];
+            wsum[w] = tot;
+            tot += v;
+        }
+        wbase = tot > 0 ? atomicAdd(ngroups, tot) : 0;
+    }
+    __syncthreads();
+    const int base = wbase + wsum[wave] + incl - g;
+    for (int k = 0; k < g; ++k) groups[base + k] = make_int2((int)b, k);
+}
+
+static constexpr int DENSE_PER = 4;  // particles per thread and round
+
+// persistent: workgroup w takes entries w, w + grid, ... of the work list (full groups are
+// alike, so a static share balances); each round's loads are issued before the previous
+// round's terms are added
+template <typename ACC>
+__global__ __launch_bounds__(DEP_WG) void deposit_dense_kernel(DepArgs A, const int2 *__restrict__ groups,
+                                                               const int *__restrict__ ngroups) {
+    constexpr int B = 8, NWE = B + 1, NW3 = NWE * NWE * NWE;
+    constexpr bool FX = sizeof(ACC) == 8;
+    constexpr int GP = DEP_MERGE * ITEM_P;
+    constexpr int RP = DENSE_PER * DEP_WG;  // particles per round
+    __shared__ unsigned long long nodes[NW3];
+    const int N = A.g.N, NB = A.g.NB;
+    const float q = A.mass_nv;
+    const int total = ngroups[0];
+    for (int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+        for (int i = threadIdx.x; i < NW3; i += DEP_WG) nodes[i] = 0ull;
+        const int2 gr = groups[gi];
+        const int b = gr.x;
+        const int bstart = A.start[b], bend = A.start[b + 1];
+        const int p0g = bstart + gr.y * GP, end = min(bend, p0g + GP);
+        const int ox = (b % NB) * B, oy = ((b / NB) % NB) * B, oz = (int)(b / ((int64_t)NB * NB)) * B;
+        const double sc = FX ? 1.0 : bound_scale(end - p0g, q);
+        int c[DENSE_PER];
+        float ux[DENSE_PER], uy[DENSE_PER], uz[DENSE_PER];
+        auto fetch = [&](int p0) {
+#pragma unroll
+            for (int k = 0; k < DENSE_PER; ++k) {
+                const int p = p0 + threadIdx.x + k * DEP_WG;
+                c[k] = -1;
+                if (p < end) {
+                    c[k] = A.s.cell[p];
+                    ux[k] = A.s.off[0][p];
+                    uy[k] = A.s.off[1][p];
+                    uz[k] = A.s.off[2][p];
+                }
+            }
+        };
+        fetch(p0g);
+        __syncthreads();  // the window is cleared
+        for (int p0 = p0g; p0 < end; p0 += RP) {
+            int cc[DENSE_PER];
+            float vx[DENSE_PER], vy[DENSE_PER], vz[DENSE_PER];
+#pragma unroll
+            for (int k = 0; k < DENSE_PER; ++k) {
+                cc[k] = c[k];
+                vx[k] = ux[k];
+                vy[k] = uy[k];
+                vz[k] = uz[k];
+            }
+            if (p0 + RP < end) fetch(p0 + RP);
+#pragma unroll
+            for (int k = 0; k < DENSE_PER; ++k) {
+                if (cc[k] < 0) continue;
+                int x, y, z;
+                decode_cell(cc[k], A.g, x, y, z);
+                const int base = (x & (B - 1)) + NWE * ((y & (B - 1)) + NWE * (z & (B - 1)));
+                const float wx[2] = {1.0f - vx[k], vx[k]}, wy[2] = {1.0f - vy[k], vy[k]},
+                            wz[2] = {1.0f - vz[k], vz[k]};
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+                    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                        for (int cx = 0; cx < 2; ++cx)
+                            atomicAdd(&nodes[base + cx + NWE * (cy + NWE * cz)],
+                                      dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], sc, A.fxs[0]));
+            }
+        }
+        __syncthreads();
+        const double inv = 1.0 / sc;
+        for (int i = threadIdx.x; i < NW3; i += DEP_WG) {
+            const unsigned long long w = nodes[i];
+            if (w == 0ull) continue;
+            const int ax = i % NWE, ay = (i / NWE) % NWE, az = i / (NWE * NWE);
+            const int64_t idx = site_index(N, A.h, wrapi(ox + ax, N), wrapi(oy + ay, N), wrapi(oz + az, N));
+            if constexpr (FX) AccOps<ACC>::flush(A, idx, w);
+            else atomicAdd(&A.field[idx], (float)((double)(long long)w * inv));
+        }
+        __syncthreads();  // the window is reused
+    }
+}
+
 // maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
 // non-negative floats: the bound of the vector and tensor fixed-point deposits
 __global__ void dep_max_kernel(int64_t np, Soa32 s, float mass_nv, unsigned int *__restrict__ out) {
@@ -1970,8 +2287,12 @@ struct lf2mi_parts_s {
     int *bstart, *bcount, *istart, *bsum, *item_bucket;
     int4 *chunks;  // gather chunks of sparse bricks (at most one per brick)
     int *nchunk;   // [number of chunks, claim counter]
+    int2 *dgroups; // work list of the dense deposit kernel (at most cap / group + bricks)
+    int *ndgroup;  // [number of entries, unused]
     unsigned long long *fx;
     int64_t fx_n;
+    void *shell;          // shell and seam buffers of the tiles deposit kernel (grown on demand)
+    int64_t shell_bytes;
     unsigned int *vmax_f32;
     unsigned long long *vmax_f64;
     double *red;   // statistics: RED_BLOCKS x RED_Q partials, then RED_Q results
@@ -2062,6 +2383,57 @@ static void launch_deposit(const DepArgs &A, bool b8, unsigned grid, hipStream_t
         if (pm) deposit_f32_kernel<KIND, ACC, 0, true><<<grid, DEP_WG, 0, st>>>(A);
         else deposit_f32_kernel<KIND, ACC, 0, false><<<grid, DEP_WG, 0, st>>>(A);
     }
+}
+
+// bricks with fewer particles are deposited by deposit_tiles_kernel, the others by
+// deposit_dense_kernel
+static constexpr int DENSE_MIN = 192;
+// persistent workgroups of the dense deposit kernel
+static constexpr int DENSE_WORKERS = 2048;
+
+// the scalar deposit of a global mass into a periodic target with 8^3 bricks
+static bool tiles_path(const lf2mi_parts_s *p, int kind, int target, bool pm) {
+    return kind == LF2MI_PROJ_SCALAR && !pm && target == LF2MI_DEP_PERIODIC && p->g.B == 8 &&
+           p->g.NB % TILE_B == 0 && p->N <= 1024;
+}
+
+// the shell and seam buffers of the tiles kernel (grown on demand): N^3 (9 + 8) / 64 values
+// and 81 per tile
+static int shell_buffers(lf2mi_parts_s *p, size_t esz, void **ysh, void **zsh, void **xsh) {
+    const int64_t N = p->N, NB = p->g.NB;
+    const int64_t ny = N * N * N / 64 * 9, nz = N * N * N / 64 * 8, nx = NB * NB * (NB / TILE_B) * 81;
+    const int64_t bytes = (ny + nz + nx) * (int64_t)esz;
+    if (p->shell_bytes < bytes) {
+        if (p->shell) hipFree(p->shell);
+        p->shell = nullptr;
+        p->shell_bytes = 0;
+        LF2MI_HIP(hipMalloc(&p->shell, (size_t)bytes));
+        p->shell_bytes = bytes;
+    }
+    *ysh = p->shell;
+    *zsh = (char *)p->shell + ny * (int64_t)esz;
+    *xsh = (char *)p->shell + (ny + nz) * (int64_t)esz;
+    return LF2MI_OK;
+}
+
+template <typename ACC>
+static int launch_tiles_dense(lf2mi_parts_s *p, const DepArgs &A, bool overwrite, hipStream_t st) {
+    void *ysh = nullptr, *zsh = nullptr, *xsh = nullptr;
+    if (int rc = shell_buffers(p, sizeof(ACC), &ysh, &zsh, &xsh)) return rc;
+    static_assert(DENSE_MIN <= 64 * 3, "three particles per lane");
+    const int64_t NB = p->g.NB, N = p->N;
+    deposit_tiles_kernel<ACC><<<(unsigned)(NB * NB * (NB / TILE_B)), 64 * TILE_B, 0, st>>>(
+        A, DENSE_MIN, overwrite ? 1 : 0, (ACC *)ysh, (ACC *)zsh, (ACC *)xsh);
+    LF2MI_LAUNCHED();
+    if (int rc = fill_int(p->ndgroup, 2, 0, st)) return rc;
+    dense_groups_kernel<<<grid1(p->g.nbk, DG_WG), DG_WG, 0, st>>>(p->bstart, p->g.nbk, DENSE_MIN, p->dgroups, p->ndgroup);
+    LF2MI_LAUNCHED();
+    deposit_dense_kernel<ACC><<<DENSE_WORKERS, DEP_WG, 0, st>>>(A, p->dgroups, p->ndgroup);
+    LF2MI_LAUNCHED();
+    const int64_t nfold = N * NB * N + N * (N - NB) * NB + (NB / TILE_B) * (N - NB) * (N - NB);
+    deposit_fold_kernel<ACC><<<grid1(nfold), 256, 0, st>>>(A, (const ACC *)ysh, (const ACC *)zsh, (const ACC *)xsh);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
 }
 
 static int ic_common(lf2mi_parts_s *p, IcArgs &A, hipStream_t st) {
@@ -2218,6 +2590,8 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         A((void **)&p->vmax_f32, sizeof(unsigned int));
         A((void **)&p->chunks, (size_t)p->g.nbk * sizeof(int4));
         A((void **)&p->nchunk, 2 * sizeof(int));
+        A((void **)&p->dgroups, (size_t)(cap / (DEP_MERGE * ITEM_P) + p->g.nbk + 1) * sizeof(int2));
+        A((void **)&p->ndgroup, 2 * sizeof(int));
     } else {
         Soa64 &s = p->b64;
         A((void **)&s.cell, cap * sizeof(int32_t));
@@ -2250,6 +2624,7 @@ int lf2mi_parts_destroy(lf2mi_parts p) {
     if (!p) return LF2MI_OK;
     for (void *q : p->allocs) hipFree(q);
     if (p->fx) hipFree(p->fx);
+    if (p->shell) hipFree(p->shell);
     delete p;
     return LF2MI_OK;
 }
@@ -2763,7 +3138,12 @@ int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha, do
     return rc;
 }
 
-int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, void *stream) {
+}  // extern "C"
+
+// overwrite: the native scalar deposit on the tiles path stores every interior node instead of
+// adding to it (the PM step's deposit into a field it would otherwise clear first)
+static int deposit_impl(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, bool overwrite,
+                        void *stream) {
     if (!p || !field) return fail(LF2MI_EINVAL, "lf2mi_deposit: null argument");
     if (kind < LF2MI_PROJ_SCALAR || kind > LF2MI_PROJ_SYMTENSOR_NGP)
         return fail(LF2MI_EINVAL, "bad projection kind %d", kind);
@@ -2878,12 +3258,17 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
                                                    "deposit_f32_symtensor_ngp_fixed64"};
             LF2MI_PROF(namesfx[kind], st);
             const bool b8 = p->g.B == 8;
-            switch (kind) {
-                case 0: launch_deposit<0, unsigned long long>(A, b8, g, st); break;
-                case 1: launch_deposit<1, unsigned long long>(A, b8, g, st); break;
-                case 2: launch_deposit<2, unsigned long long>(A, b8, g, st); break;
-                case 3: launch_deposit<3, unsigned long long>(A, b8, g, st); break;
-                default: launch_deposit<4, unsigned long long>(A, b8, g, st); break;
+            if (tiles_path(p, kind, target, pm)) {
+                // the fixed-point buffer starts cleared, so its interior can be stored
+                if (int rc = launch_tiles_dense<unsigned long long>(p, A, true, st)) return rc;
+            } else {
+                switch (kind) {
+                    case 0: launch_deposit<0, unsigned long long>(A, b8, g, st); break;
+                    case 1: launch_deposit<1, unsigned long long>(A, b8, g, st); break;
+                    case 2: launch_deposit<2, unsigned long long>(A, b8, g, st); break;
+                    case 3: launch_deposit<3, unsigned long long>(A, b8, g, st); break;
+                    default: launch_deposit<4, unsigned long long>(A, b8, g, st); break;
+                }
             }
             LF2MI_LAUNCHED();
         }
@@ -2903,6 +3288,7 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
                                            "deposit_f32_vector_ngp", "deposit_f32_symtensor_ngp"};
     LF2MI_PROF(names32[kind], st);
     const bool b8 = p->g.B == 8;
+    if (tiles_path(p, kind, target, pm)) return launch_tiles_dense<float>(p, A, overwrite, st);
     switch (kind) {
         case 0: launch_deposit<0, float>(A, b8, g, st); break;
         case 1: launch_deposit<1, float>(A, b8, g, st); break;
@@ -2912,6 +3298,12 @@ int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, in
     }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
+}
+
+extern "C" {
+
+int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, void *stream) {
+    return deposit_impl(p, kind, field, halo, target, acc, false, stream);
 }
 
 int lf2mi_gather_kick_drift(lf2mi_parts p, const void *phi, int halo, double dt_kick, double dt_drift, double *vmax2,
@@ -3048,8 +3440,15 @@ int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int 
     if (!p || !plan || !rho || !phi) return fail(LF2MI_EINVAL, "lf2mi_pm_step: null argument");
     const int N = p->N;
     const int64_t G = N + 2 * halo;
-    if (int rc = lf2mi_zero(rho, G * G * G, p->dtype, stream)) return rc;
-    if (int rc = lf2mi_deposit(p, LF2MI_PROJ_SCALAR, rho, halo, LF2MI_DEP_PERIODIC, acc, stream)) return rc;
+    // the native tiles-path deposit stores every interior node: only the ghosts need clearing
+    const bool store = p->dtype == LF2MI_F32 && acc == LF2MI_ACC_NATIVE && halo >= 1 &&
+                       tiles_path(p, LF2MI_PROJ_SCALAR, LF2MI_DEP_PERIODIC, p->mass_src && p->b32[p->cur].mass);
+    if (store) {
+        if (int rc = zero_ghosts(rho, N, halo, 1, p->dtype, as_stream(stream))) return rc;
+    } else if (int rc = lf2mi_zero(rho, G * G * G, p->dtype, stream)) {
+        return rc;
+    }
+    if (int rc = deposit_impl(p, LF2MI_PROJ_SCALAR, rho, halo, LF2MI_DEP_PERIODIC, acc, store, stream)) return rc;
     // box volume 1: the mean density is the total mass
     const double scale = 1.0 / ((double)N * N * N);
     if (int rc = lf2mi_poisson_solve(plan, rho, phi, p->total_mass, scale, stream)) return rc;
