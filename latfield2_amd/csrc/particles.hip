@@ -1142,7 +1142,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
 }
 
 // ---------------------------------------------------------------- scalar deposit, periodic target
-// Four kernels replace the item kernels when the target is periodic, the brick edge 8, the
+// Three kernels replace the item kernels when the target is periodic, the brick edge 8, the
 // number of bricks per row a multiple of TILE_B and the mass global (the PM step's deposit).
 // A brick's nodes are [0,8)^3 relative to its origin; its particles also reach the first
 // node layer of the bricks above it along x, y and z.
@@ -1153,14 +1153,16 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
 //    then written once, TILE_X consecutive x per row (stored, or added to the field).  The layer
 //    that belongs to the bricks above in y or z goes to two shell buffers and the tile's last
 //    x = 8 plane to a seam buffer, one value per node.  No global atomics, no sort.
-//  * dense_groups_kernel + deposit_dense_kernel: the bricks with dense_min or more particles,
+//  * dense_groups_kernel + dense_body: the bricks with dense_min or more particles,
 //    in groups of up to DEP_MERGE items (a work list shared out statically among persistent
 //    workgroups); each particle's 8 terms go straight into a 9^3 LDS window (no sort: a
 //    wave's 64 particles of a crowded brick rarely share a cell), added to the field with one
 //    global atomic per non-zero node after the tiles kernel's stores.
-//  * deposit_fold_kernel: every node in the lowest y or z layer of a brick, or in the first
-//    x plane of a tile, adds the shell and seam values meant for it (plain read-modify-write,
-//    one thread per node, x fastest).
+//  * fold_node: every node in the lowest y or z layer of a brick, or in the first x plane of a
+//    tile, adds the shell and seam values meant for it, one thread per node, x fastest.  The
+//    first FOLD_BLOCKS workgroups of the dense kernel's launch do it with global atomics
+//    while the others deposit the dense bricks (deposit_dense_fold_kernel): one is memory
+//    bound, the other LDS bound.
 // Window sums are exact 64-bit fixed point: per term a power-of-two scale 2^(51 - e) of a
 // bound n q < 2^e (ACC float: n = dense_min for the tiles, whose x = 0 planes then take at
 // most twice that, the group's count for the dense kernel), or the term quantised on its own
@@ -1168,10 +1170,17 @@ __global__ __launch_bounds__(DEP_WG) void deposit_merged_kernel(DepArgs A) {
 static constexpr int TILE_B = 8;           // bricks per tile (one wave each)
 static constexpr int TILE_X = 8 * TILE_B;  // nodes per tile row
 
+// a particle's term for node weight w: float accumulation takes q w times the window scale
+// in FP64 (qsc = q * scale, one FMA against 1.5 2^52 as in to_window); the fixed point
+// quantises fl(q w) to the global scale on its own
 template <typename ACC>
-__device__ __forceinline__ unsigned long long dep_term(float q, float w, double sc, double fxs) {
-    if constexpr (sizeof(ACC) == 8) return AccOps<ACC>::term(q, w, fxs);
-    else return to_window(q * w, sc);
+__device__ __forceinline__ unsigned long long dep_term(float q, float w, double qsc, double fxs) {
+    if constexpr (sizeof(ACC) == 8) {
+        return AccOps<ACC>::term(q, w, fxs);
+    } else {
+        const double t = fma((double)w, qsc, 0x1.8p52);
+        return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000ll);
+    }
 }
 
 // scale of a window whose bound is n particles of value q (float accumulation)
@@ -1194,9 +1203,10 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
                                                                     ACC *__restrict__ xsh) {
     constexpr int B = 8, NWE = 9, NW3 = NWE * NWE * NWE, PL = NWE * NWE, SP = 3;
     constexpr bool FX = sizeof(ACC) == 8;
+    static_assert(TILE_X == 64, "one lane per x of the tile");
     __shared__ unsigned long long win[TILE_B][NW3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int NB = A.g.NB, N = A.g.N;
+    const int NB = A.g.NB, N = A.g.N, G = N + 2 * A.h;
     const int ntile = NB / TILE_B;
     const int tile = blockIdx.x % ntile, row = blockIdx.x / ntile;
     const int by = row % NB, bz = row / NB;
@@ -1204,7 +1214,7 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
     unsigned long long *w = win[wave];
     for (int i = lane; i < NW3; i += 64) w[i] = 0ull;
     const float q = A.mass_nv;
-    const double sc = FX ? 1.0 : bound_scale(dense_min, q);
+    const double sc = FX ? 1.0 : bound_scale(dense_min, q), qsc = (double)q * sc, inv = 1.0 / sc;
     const int a = A.start[brick], n = A.start[brick + 1] - a;
     if (n < dense_min) {
         int c[SP];
@@ -1234,7 +1244,7 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
 #pragma unroll
                     for (int cx = 0; cx < 2; ++cx)
                         atomicAdd(&w[base + cx + NWE * (cy + NWE * cz)],
-                                  dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], sc, A.fxs[0]));
+                                  dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], qsc, A.fxs[0]));
         }
     }
     __syncthreads();
@@ -1242,33 +1252,35 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
     if (wave > 0)
         for (int yz = lane; yz < PL; yz += 64) w[NWE * yz] += win[wave - 1][B + NWE * yz];
     __syncthreads();
-    const double inv = 1.0 / sc;
     auto cvt = [&](unsigned long long v) -> ACC {
         if constexpr (FX) return v;
         else return (float)((double)(long long)v * inv);
     };
     const int X0 = tile * TILE_X;
-    // own nodes, TILE_X consecutive x per row
-    for (int t = threadIdx.x; t < TILE_X * B * B; t += 64 * TILE_B) {
-        const int xi = t % TILE_X, yz = t / TILE_X, y = yz % B, z = yz / B;
-        const unsigned long long v = win[xi / B][(xi % B) + NWE * (y + NWE * z)];
-        const int64_t idx = site_index(N, A.h, X0 + xi, by * B + y, bz * B + z);
-        if constexpr (FX) {
-            if (set) A.fx[idx] = v;
-            else if (v) A.fx[idx] += v;
-        } else {
-            const float f = cvt(v);
-            if (set) A.field[idx] = f;
-            else if (v) A.field[idx] += f;
+    // own nodes: lane = x within the tile, wave = y, rows z = 0 .. 7
+    {
+        const int64_t i0 = site_index(N, A.h, X0 + lane, by * B + wave, bz * B);
+        const unsigned long long *src = &win[lane / B][(lane % B) + NWE * wave];
+#pragma unroll
+        for (int z = 0; z < B; ++z) {
+            const unsigned long long v = src[NWE * NWE * z];
+            const int64_t idx = i0 + (int64_t)z * G * G;
+            if constexpr (FX) {
+                if (set) A.fx[idx] = v;
+                else if (v) A.fx[idx] += v;
+            } else {
+                const float f = cvt(v);
+                if (set) A.field[idx] = f;
+                else if (v) A.field[idx] += f;
+            }
         }
     }
-    // y = 8 layer (z in 0..8), then z = 8 layer (y in 0..7)
-    for (int t = threadIdx.x; t < TILE_X * (NWE + B); t += 64 * TILE_B) {
-        const int xi = t % TILE_X, r = t / TILE_X;
+    // y = 8 layer (rows z = 0 .. 8), then z = 8 layer (rows y = 0 .. 7): row r = wave + 8 j
+    for (int r = wave; r < NWE + B; r += TILE_B) {
         const int y = r < NWE ? B : r - NWE, z = r < NWE ? r : B;
-        const ACC v = cvt(win[xi / B][(xi % B) + NWE * (y + NWE * z)]);
-        if (r < NWE) ysh[((int64_t)row * NWE + z) * N + X0 + xi] = v;
-        else zsh[((int64_t)row * B + y) * N + X0 + xi] = v;
+        const ACC v = cvt(win[lane / B][(lane % B) + NWE * (y + NWE * z)]);
+        if (r < NWE) ysh[((int64_t)row * NWE + z) * N + X0 + lane] = v;
+        else zsh[((int64_t)row * B + y) * N + X0 + lane] = v;
     }
     // the tile's last x = 8 plane, for the next tile's first plane
     const int nt = tile + 1 == ntile ? 0 : tile + 1;
@@ -1279,14 +1291,14 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
 // one thread per node that receives shell or seam values, x fastest: the lowest y layer of
 // each brick (N x NB x N nodes), then the lowest z layer without it (N x 7 NB x NB), then the
 // first x plane of each tile without both (ntile x 7 NB x 7 NB, x slowest)
+// (with global atomics, so that the dense deposit may run at the same time)
 template <typename ACC>
-__global__ __launch_bounds__(256) void deposit_fold_kernel(DepArgs A, const ACC *__restrict__ ysh,
-                                                           const ACC *__restrict__ zsh, const ACC *__restrict__ xsh) {
+__device__ __forceinline__ void fold_node(const DepArgs &A, const ACC *__restrict__ ysh, const ACC *__restrict__ zsh,
+                                          const ACC *__restrict__ xsh, unsigned t) {
     constexpr int B = 8, NWE = 9, PL = NWE * NWE;
     constexpr bool FX = sizeof(ACC) == 8;
     const unsigned N = A.g.N, NB = A.g.NB, ntile = NB / TILE_B;
     const unsigned ny = N * NB * N, nz = N * (N - NB) * NB, nx = ntile * (N - NB) * (N - NB);
-    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ny + nz + nx) return;
     // divisions by N and NB as shifts when N is a power of two (wave-uniform branch)
     const int lg = A.g.lgN;
@@ -1331,9 +1343,15 @@ __global__ __launch_bounds__(256) void deposit_fold_kernel(DepArgs A, const ACC 
     }
     if (v == ACC(0)) return;
     const int64_t idx = site_index((int)N, A.h, (int)X, (int)Y, (int)Z);
-    if constexpr (FX) A.fx[idx] += v;
-    else A.field[idx] += v;
+    if constexpr (FX) atomicAdd(&A.fx[idx], v);
+    else atomicAdd(&A.field[idx], v);
 }
+
+// number of nodes fold_node serves
+__host__ __device__ inline unsigned fold_nodes(unsigned N, unsigned NB) {
+    return N * NB * N + N * (N - NB) * NB + (NB / TILE_B) * (N - NB) * (N - NB);
+}
+
 
 // work list of the dense kernel: (brick, group) pairs of the bricks with dense_min or more
 // particles, a group being up to DEP_MERGE * ITEM_P consecutive particles of the brick
@@ -1380,8 +1398,8 @@ static constexpr int DENSE_PER = 4;  // particles per thread and round
 // alike, so a static share balances); each round's loads are issued before the previous
 // round's terms are added
 template <typename ACC>
-__global__ __launch_bounds__(DEP_WG) void deposit_dense_kernel(DepArgs A, const int2 *__restrict__ groups,
-                                                               const int *__restrict__ ngroups) {
+__device__ __forceinline__ void dense_body(const DepArgs &A, const int2 *__restrict__ groups,
+                                           const int *__restrict__ ngroups, int bid, int nbid) {
     constexpr int B = 8, NWE = B + 1, NW3 = NWE * NWE * NWE;
     constexpr bool FX = sizeof(ACC) == 8;
     constexpr int GP = DEP_MERGE * ITEM_P;
@@ -1390,14 +1408,14 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_kernel(DepArgs A, const 
     const int N = A.g.N, NB = A.g.NB;
     const float q = A.mass_nv;
     const int total = ngroups[0];
-    for (int gi = blockIdx.x; gi < total; gi += gridDim.x) {
+    for (int gi = bid; gi < total; gi += nbid) {
         for (int i = threadIdx.x; i < NW3; i += DEP_WG) nodes[i] = 0ull;
         const int2 gr = groups[gi];
         const int b = gr.x;
         const int bstart = A.start[b], bend = A.start[b + 1];
         const int p0g = bstart + gr.y * GP, end = min(bend, p0g + GP);
         const int ox = (b % NB) * B, oy = ((b / NB) % NB) * B, oz = (int)(b / ((int64_t)NB * NB)) * B;
-        const double sc = FX ? 1.0 : bound_scale(end - p0g, q);
+        const double sc = FX ? 1.0 : bound_scale(end - p0g, q), qsc = (double)q * sc;
         int c[DENSE_PER];
         float ux[DENSE_PER], uy[DENSE_PER], uz[DENSE_PER];
         auto fetch = [&](int p0) {
@@ -1441,7 +1459,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_kernel(DepArgs A, const 
 #pragma unroll
                         for (int cx = 0; cx < 2; ++cx)
                             atomicAdd(&nodes[base + cx + NWE * (cy + NWE * cz)],
-                                      dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], sc, A.fxs[0]));
+                                      dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], qsc, A.fxs[0]));
             }
         }
         __syncthreads();
@@ -1456,6 +1474,24 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_kernel(DepArgs A, const 
         }
         __syncthreads();  // the window is reused
     }
+}
+
+// the first `fold_blocks` workgroups add the shells with atomics, the others run the dense
+// deposit: the two overlap (one memory bound, the other LDS bound)
+template <typename ACC>
+__global__ __launch_bounds__(DEP_WG) void deposit_dense_fold_kernel(DepArgs A, const int2 *__restrict__ groups,
+                                                                    const int *__restrict__ ngroups,
+                                                                    const ACC *__restrict__ ysh,
+                                                                    const ACC *__restrict__ zsh,
+                                                                    const ACC *__restrict__ xsh, int fold_blocks) {
+    static_assert(DEP_WG == 256, "workgroup size of both parts");
+    if ((int)blockIdx.x < fold_blocks) {
+        const unsigned n = fold_nodes(A.g.N, A.g.NB), stride = (unsigned)fold_blocks * DEP_WG;
+        for (unsigned t = blockIdx.x * DEP_WG + threadIdx.x; t < n; t += stride)
+            fold_node<ACC>(A, ysh, zsh, xsh, t);
+        return;
+    }
+    dense_body<ACC>(A, groups, ngroups, (int)blockIdx.x - fold_blocks, (int)gridDim.x - fold_blocks);
 }
 
 // maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
@@ -2390,6 +2426,8 @@ static void launch_deposit(const DepArgs &A, bool b8, unsigned grid, hipStream_t
 static constexpr int DENSE_MIN = 192;
 // persistent workgroups of the dense deposit kernel
 static constexpr int DENSE_WORKERS = 2048;
+// workgroups of the dense deposit's launch that add the tiles kernel's shells meanwhile
+static constexpr int FOLD_BLOCKS = 1024;
 
 // the scalar deposit of a global mass into a periodic target with 8^3 bricks
 static bool tiles_path(const lf2mi_parts_s *p, int kind, int target, bool pm) {
@@ -2428,10 +2466,8 @@ static int launch_tiles_dense(lf2mi_parts_s *p, const DepArgs &A, bool overwrite
     if (int rc = fill_int(p->ndgroup, 2, 0, st)) return rc;
     dense_groups_kernel<<<grid1(p->g.nbk, DG_WG), DG_WG, 0, st>>>(p->bstart, p->g.nbk, DENSE_MIN, p->dgroups, p->ndgroup);
     LF2MI_LAUNCHED();
-    deposit_dense_kernel<ACC><<<DENSE_WORKERS, DEP_WG, 0, st>>>(A, p->dgroups, p->ndgroup);
-    LF2MI_LAUNCHED();
-    const int64_t nfold = N * NB * N + N * (N - NB) * NB + (NB / TILE_B) * (N - NB) * (N - NB);
-    deposit_fold_kernel<ACC><<<grid1(nfold), 256, 0, st>>>(A, (const ACC *)ysh, (const ACC *)zsh, (const ACC *)xsh);
+    deposit_dense_fold_kernel<ACC><<<FOLD_BLOCKS + DENSE_WORKERS, DEP_WG, 0, st>>>(
+        A, p->dgroups, p->ndgroup, (const ACC *)ysh, (const ACC *)zsh, (const ACC *)xsh, FOLD_BLOCKS);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
