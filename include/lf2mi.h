@@ -159,6 +159,16 @@ int lf2mi_fft_execute(lf2mi_fft_plan plan, int direction, void *rfield, void *kf
 int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, double scale,
                         int dtype, void *stream);
 
+/* lf2mi_poisson_green without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The eigenvalue table lives in a static device array of
+ * 1024 doubles (8 KiB per GPU, held until the process ends), so N <= 1024 (LF2MI_ERANGE
+ * otherwise); no hipMalloc/hipFree, no host synchronisation.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (different streams) race.  Results equal
+ * lf2mi_poisson_green's to FP64 rounding (the table is computed on the device). */
+int lf2mi_poisson_green_noalloc(void *kfield, int N, int C, int khalo, double res, double scale,
+                                int dtype, void *stream);
+
 /* Fused Poisson solve on the plan's geometry (C must be 1):
  *   phi = C2R( scale * G(k) * R2C(rho - shift) ),  G = -1/K^2, G(0) = 0,
  *   K^2 = sum_i N^2 (2 - 2 cos(2 pi k_i / N))

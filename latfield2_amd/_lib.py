@@ -57,6 +57,7 @@ SIGNATURES = {
     "lf2mi_fft_plan_bytes": [_vp, _i64p],
     "lf2mi_fft_execute": [_vp, _i, _vp, _vp, _vp],
     "lf2mi_poisson_green": [_vp, _i, _i, _i, _d, _d, _i, _vp],
+    "lf2mi_poisson_green_noalloc": [_vp, _i, _i, _i, _d, _d, _i, _vp],
     "lf2mi_poisson_solve": [_vp, _vp, _vp, _d, _d, _vp],
     "lf2mi_parts_create": [_i, _i64, _i, _i, ctypes.POINTER(_vp)],
     "lf2mi_parts_destroy": [_vp],

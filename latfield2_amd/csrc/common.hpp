@@ -24,6 +24,10 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// looks up, once per device, the current device's address of the scratch table of
+// lf2mi_poisson_green_noalloc (fft.hip)
+int green_scratch_init();
+
 // clears the ghost sites of a cubic field (field.hip)
 int zero_ghosts(void *field, int N, int h, int C, int dtype, hipStream_t st);
 

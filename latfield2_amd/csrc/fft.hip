@@ -942,6 +942,37 @@ __global__ void green_kernel(cplx<T> *__restrict__ k, int N, int C, int kh, doub
     }
 }
 
+// ---------------------------------------------------------------- Green's multiply, no allocation
+// `scratch` holds the eigenvalue table s4[q] = 4 sin^2(pi q / N) of lf2mi_poisson_green_noalloc
+// (at most GREEN_SCRATCH_N entries, 8 KiB).  Each GPU the library runs on holds its own copy
+// until the process ends; every call on a GPU overwrites it, so calls that overlap on one GPU
+// (on different streams) race.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.
+static constexpr int GREEN_SCRATCH_N = 1024;
+static constexpr int GREEN_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[GREEN_SCRATCH_N];
+static double *g_scratch_addr[GREEN_MAX_DEVICES];  // device address of `scratch`, per device
+
+// the scratch table's address on the current device, looked up once per device
+int green_scratch_init() {
+    int dev = 0;
+    LF2MI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= GREEN_MAX_DEVICES)
+        return fail(LF2MI_ERANGE, "device %d: the library keeps scratch addresses for %d devices", dev, GREEN_MAX_DEVICES);
+    if (g_scratch_addr[dev]) return LF2MI_OK;
+    void *ptr = nullptr;
+    LF2MI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_addr[dev] = static_cast<double *>(ptr);
+    return LF2MI_OK;
+}
+
+__global__ void green_table_kernel(double *__restrict__ s4, int N) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    const double sh = sin(3.141592653589793 * (double)q / (double)N);
+    s4[q] = 4.0 * sh * sh;
+}
+
 }  // namespace lf2mi
 
 using namespace lf2mi;
@@ -1102,6 +1133,32 @@ int lf2mi_poisson_green(void *kfield, int N, int C, int khalo, double res, doubl
     }
     LF2MI_LAUNCHED();
     LF2MI_HIP(hipFreeAsync(s4v, st));
+    return LF2MI_OK;
+}
+
+int lf2mi_poisson_green_noalloc(void *kfield, int N, int C, int khalo, double res, double scale, int dtype,
+                                void *stream) {
+    if (!kfield || N < 1 || C < 1 || khalo < 0 || !(res > 0))
+        return fail(LF2MI_EINVAL, "lf2mi_poisson_green_noalloc: bad argument");
+    if (dtype != LF2MI_F32 && dtype != LF2MI_F64) return fail(LF2MI_EINVAL, "bad dtype %d", dtype);
+    if (N > GREEN_SCRATCH_N)
+        return fail(LF2MI_ERANGE, "lf2mi_poisson_green_noalloc: N = %d exceeds its %d-entry scratch table", N,
+                    GREEN_SCRATCH_N);
+    if (int rc = green_scratch_init()) return rc;
+    int dev = 0;
+    LF2MI_HIP(hipGetDevice(&dev));
+    double *s4v = g_scratch_addr[dev];
+    hipStream_t st = as_stream(stream);
+    green_table_kernel<<<(N + 255) / 256, 256, 0, st>>>(s4v, N);
+    LF2MI_LAUNCHED();
+    const int64_t total = (int64_t)(N / 2 + 1) * N * N;
+    const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+    LF2MI_PROF("poisson_green", st);
+    if (dtype == LF2MI_F32)
+        green_kernel<float><<<g, 256, 0, st>>>((float2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, s4v);
+    else
+        green_kernel<double><<<g, 256, 0, st>>>((double2 *)kfield, N, C, khalo, 1.0 / (res * res), scale, s4v);
+    LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
 

@@ -264,7 +264,7 @@ const char *lf2mi_last_error(void) { return g_last_error.c_str(); }
 int lf2mi_device_init(int device) {
     LF2MI_HIP(hipSetDevice(device));
     LF2MI_HIP(hipFree(nullptr));
-    return LF2MI_OK;
+    return green_scratch_init();
 }
 
 int lf2mi_synchronize(void) {

@@ -63,6 +63,11 @@ def test_host_side_argument_checks():
     assert L.lf2mi_ghost_fill(None, 8, 1, 1, 0, None) == -1
     assert L.lf2mi_ghost_fill(ctypes.c_void_p(1), 8, 9, 1, 0, None) == -1  # halo larger than N
     assert L.lf2mi_poisson_green(None, 8, 1, 0, 0.1, 1.0, 0, None) == -1
+    assert L.lf2mi_poisson_green_noalloc(None, 8, 1, 0, 0.1, 1.0, 0, None) == -1
+    # the no-allocation Green's multiply has a 1024-entry table: larger N is refused before any
+    # device work
+    assert L.lf2mi_poisson_green_noalloc(ctypes.c_void_p(1), 2048, 1, 0, 0.1, 1.0, 0, None) == -4
+    assert b"exceeds" in L.lf2mi_last_error()
     assert L.lf2mi_deposit(None, 0, None, 1, 0, 0, None) == -1
     h = ctypes.c_void_p()
     assert L.lf2mi_parts_create(1, 10, 0, 0, ctypes.byref(h)) == -4
