@@ -335,13 +335,15 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
 // ---------------------------------------------------------------- x pass, R2C
 // Block = P row pairs x C components = P*C complex lines; line (p, c) carries rows
 // (2p, 2p+1) of component c as re/im.  Row r = y + N z of the real interior.
-template <typename T, int N, int L, int WG>
+// CC: the component count when it is known at compile time (1: the Poisson solve's field),
+// else 0 and A.C
+template <typename T, int N, int L, int WG, int CC>
 __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
     using V = cplx<T>;
     constexpr int ES = L + 1;
     extern __shared__ unsigned char smem_raw[];
     V *lds = reinterpret_cast<V *>(smem_raw);
-    const int C = A.C;
+    const int C = CC ? CC : A.C;
     const int P = L / C;
     const int64_t pair0 = (int64_t)blockIdx.x * P;
     const int64_t npairs = (int64_t)N * N / 2;
@@ -364,6 +366,31 @@ __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
     tile_fft_io<T, N, L, ES, WG, -1, 1, false, true>(lds, A.tw, rload, lstore);
     // split the two half spectra: A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = -i (Z[k] - conj Z[N-k]) / 2
     constexpr int H = N / 2 + 1;
+    if constexpr (CC == 1) {
+        // one output row at a time (its base address uniform), threads along kx
+        for (int rr = 0; rr < 2 * L; ++rr) {
+            const int p = rr >> 1, ab = rr & 1;
+            const int64_t pr = pair0 + p;
+            if (pr >= npairs) break;
+            const int64_t row = 2 * pr + ab;
+            const int y = (int)(row % N), z = (int)(row / N);
+            V *dst = A.kout + A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)y * A.k_s2;
+            for (int kx = threadIdx.x; kx < H; kx += WG) {
+                const V zk = lds[kx * ES + p];
+                const V zm = lds[(kx == 0 ? 0 : N - kx) * ES + p];
+                V r;
+                if (ab == 0) {
+                    r.x = (T)0.5 * (zk.x + zm.x);
+                    r.y = (T)0.5 * (zk.y - zm.y);
+                } else {
+                    r.x = (T)0.5 * (zk.y + zm.y);
+                    r.y = (T)0.5 * (zm.x - zk.x);
+                }
+                dst[kx] = r;
+            }
+        }
+        return;
+    }
     const int HC = H * C;
     for (int i = threadIdx.x; i < P * 2 * HC; i += WG) {
         const int q = i % HC;
@@ -390,19 +417,47 @@ __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
 }
 
 // ---------------------------------------------------------------- x pass, C2R
-template <typename T, int N, int L, int WG>
+template <typename T, int N, int L, int WG, int CC>
 __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
     using V = cplx<T>;
     constexpr int ES = L + 1;
     extern __shared__ unsigned char smem_raw[];
     V *lds = reinterpret_cast<V *>(smem_raw);
-    const int C = A.C;
+    const int C = CC ? CC : A.C;
     const int P = L / C;
     const int64_t pair0 = (int64_t)blockIdx.x * P;
     const int64_t npairs = (int64_t)N * N / 2;
     constexpr int H = N / 2 + 1;
     const int HC = H * C;
     // gather rows a, b into Z: Z[k] = A + iB (k in [0, N/2]), Z[N-k] = conj A + i conj B
+    if constexpr (CC == 1) {
+        // one row pair at a time (its base addresses uniform), threads along kx
+        for (int p = 0; p < L; ++p) {
+            const int64_t pr = pair0 + p;
+            if (pr >= npairs) break;
+            const int64_t ra = 2 * pr;
+            const int ya = (int)(ra % N), z = (int)(ra / N);
+            const V *srca = A.kin + A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)ya * A.k_s2;
+            const V *srcb = srca + A.k_s2;  // row ya + 1 of the same plane (N even)
+            for (int kx = threadIdx.x; kx < H; kx += WG) {
+                V a = srca[kx], b = srcb[kx];
+                if (kx == 0 || kx == N / 2) {
+                    a.y = 0;
+                    b.y = 0;
+                }
+                V z1;
+                z1.x = a.x - b.y;
+                z1.y = a.y + b.x;
+                lds[kx * ES + p] = z1;
+                if (kx != 0 && kx != N / 2) {
+                    V z2;
+                    z2.x = a.x + b.y;
+                    z2.y = b.x - a.y;
+                    lds[(N - kx) * ES + p] = z2;
+                }
+            }
+        }
+    } else
     for (int i = threadIdx.x; i < P * HC; i += WG) {
         const int q = i % HC;
         const int p = i / HC;
@@ -615,10 +670,13 @@ static int launch_x_tile(const XArgs<T> &A, int S, hipStream_t st) {
     const unsigned grid = (unsigned)((npairs + P - 1) / P);
     const size_t shm = sizeof(cplx<T>) * N * (L + 1);
     LF2MI_PROF(S < 0 ? "fft_x_r2c" : "fft_x_c2r", st);
-    if (S < 0)
-        fft_x_r2c_kernel<T, N, L, WG><<<grid, WG, shm, st>>>(A);
-    else
-        fft_x_c2r_kernel<T, N, L, WG><<<grid, WG, shm, st>>>(A);
+    if (S < 0) {
+        if (A.C == 1) fft_x_r2c_kernel<T, N, L, WG, 1><<<grid, WG, shm, st>>>(A);
+        else fft_x_r2c_kernel<T, N, L, WG, 0><<<grid, WG, shm, st>>>(A);
+    } else {
+        if (A.C == 1) fft_x_c2r_kernel<T, N, L, WG, 1><<<grid, WG, shm, st>>>(A);
+        else fft_x_c2r_kernel<T, N, L, WG, 0><<<grid, WG, shm, st>>>(A);
+    }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
