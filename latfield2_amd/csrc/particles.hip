@@ -1201,7 +1201,7 @@ template <typename ACC>
 __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, int dense_min, int set,
                                                                     ACC *__restrict__ ysh, ACC *__restrict__ zsh,
                                                                     ACC *__restrict__ xsh) {
-    constexpr int B = 8, NWE = 9, NW3 = NWE * NWE * NWE, PL = NWE * NWE, SP = 3;
+    constexpr int B = 8, NWE = 9, NW3 = NWE * NWE * NWE, PL = NWE * NWE, SP = 4;
     constexpr bool FX = sizeof(ACC) == 8;
     static_assert(TILE_X == 64, "one lane per x of the tile");
     __shared__ unsigned long long win[TILE_B][NW3];
@@ -1217,34 +1217,37 @@ __global__ __launch_bounds__(64 * TILE_B) void deposit_tiles_kernel(DepArgs A, i
     const double sc = FX ? 1.0 : bound_scale(dense_min, q), qsc = (double)q * sc, inv = 1.0 / sc;
     const int a = A.start[brick], n = A.start[brick + 1] - a;
     if (n < dense_min) {
-        int c[SP];
-        float ux[SP], uy[SP], uz[SP];
+        for (int j0 = 0; j0 < n; j0 += 64 * SP) {
+            int c[SP];
+            float ux[SP], uy[SP], uz[SP];
 #pragma unroll
-        for (int k = 0; k < SP; ++k) {
-            const int j = lane + 64 * k;
-            c[k] = -1;
-            if (j < n) {
-                c[k] = A.s.cell[a + j];
-                ux[k] = A.s.off[0][a + j];
-                uy[k] = A.s.off[1][a + j];
-                uz[k] = A.s.off[2][a + j];
+            for (int k = 0; k < SP; ++k) {
+                const int j = j0 + lane + 64 * k;
+                c[k] = -1;
+                if (j < n) {
+                    c[k] = A.s.cell[a + j];
+                    ux[k] = A.s.off[0][a + j];
+                    uy[k] = A.s.off[1][a + j];
+                    uz[k] = A.s.off[2][a + j];
+                }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < SP; ++k) {
-            if (c[k] < 0) continue;
-            int x, y, z;
-            decode_cell(c[k], A.g, x, y, z);
-            const int base = (x & (B - 1)) + NWE * ((y & (B - 1)) + NWE * (z & (B - 1)));
-            const float wx[2] = {1.0f - ux[k], ux[k]}, wy[2] = {1.0f - uy[k], uy[k]}, wz[2] = {1.0f - uz[k], uz[k]};
+            for (int k = 0; k < SP; ++k) {
+                if (c[k] < 0) continue;
+                int x, y, z;
+                decode_cell(c[k], A.g, x, y, z);
+                const int base = (x & (B - 1)) + NWE * ((y & (B - 1)) + NWE * (z & (B - 1)));
+                const float wx[2] = {1.0f - ux[k], ux[k]}, wy[2] = {1.0f - uy[k], uy[k]},
+                            wz[2] = {1.0f - uz[k], uz[k]};
 #pragma unroll
-            for (int cz = 0; cz < 2; ++cz)
+                for (int cz = 0; cz < 2; ++cz)
 #pragma unroll
-                for (int cy = 0; cy < 2; ++cy)
+                    for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-                    for (int cx = 0; cx < 2; ++cx)
-                        atomicAdd(&w[base + cx + NWE * (cy + NWE * cz)],
-                                  dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], qsc, A.fxs[0]));
+                        for (int cx = 0; cx < 2; ++cx)
+                            atomicAdd(&w[base + cx + NWE * (cy + NWE * cz)],
+                                      dep_term<ACC>(q, wx[cx] * wy[cy] * wz[cz], qsc, A.fxs[0]));
+            }
         }
     }
     __syncthreads();
@@ -1496,22 +1499,33 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_fold_kernel(DepArgs A, c
 
 // maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
 // non-negative floats: the bound of the vector and tensor fixed-point deposits
-__global__ void dep_max_kernel(int64_t np, Soa32 s, float mass_nv, unsigned int *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// (grid-stride, one atomic per workgroup and quantity: the four addresses see few atomics)
+static constexpr int DMAX_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void dep_max_kernel(int64_t np, Soa32 s, float mass_nv,
+                                                      unsigned int *__restrict__ out) {
+    __shared__ float wm[4][4];
     float m[4] = {0.f, 0.f, 0.f, 0.f};
-    if (i < np) {
-        m[0] = fabsf(s.mass ? s.mass[i] * mass_nv : mass_nv);
-        m[1] = fabsf(s.vel[0][i]);
-        m[2] = fabsf(s.vel[1][i]);
-        m[3] = fabsf(s.vel[2][i]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += stride) {
+        m[0] = fmaxf(m[0], fabsf(s.mass ? s.mass[i] * mass_nv : mass_nv));
+        m[1] = fmaxf(m[1], fabsf(s.vel[0][i]));
+        m[2] = fmaxf(m[2], fabsf(s.vel[1][i]));
+        m[3] = fmaxf(m[3], fabsf(s.vel[2][i]));
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m[k] = fmaxf(m[k], __shfl_xor(m[k], o));
     }
+    const int wave = threadIdx.x >> 6;
     if (lane_id() == 0)
-        for (int k = 0; k < 4; ++k) atomicMax(&out[k], __float_as_uint(m[k]));
+        for (int k = 0; k < 4; ++k) wm[wave][k] = m[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float v = 0.f;
+        for (int w = 0; w < 4; ++w) v = fmaxf(v, wm[w][threadIdx.x]);
+        atomicMax(&out[threadIdx.x], __float_as_uint(v));
+    }
 }
 
 struct FxScales {
@@ -2423,7 +2437,7 @@ static void launch_deposit(const DepArgs &A, bool b8, unsigned grid, hipStream_t
 
 // bricks with fewer particles are deposited by deposit_tiles_kernel, the others by
 // deposit_dense_kernel
-static constexpr int DENSE_MIN = 192;
+static constexpr int DENSE_MIN = 1024;
 // persistent workgroups of the dense deposit kernel
 static constexpr int DENSE_WORKERS = 2048;
 // workgroups of the dense deposit's launch that add the tiles kernel's shells meanwhile
@@ -2458,7 +2472,6 @@ template <typename ACC>
 static int launch_tiles_dense(lf2mi_parts_s *p, const DepArgs &A, bool overwrite, hipStream_t st) {
     void *ysh = nullptr, *zsh = nullptr, *xsh = nullptr;
     if (int rc = shell_buffers(p, sizeof(ACC), &ysh, &zsh, &xsh)) return rc;
-    static_assert(DENSE_MIN <= 64 * 3, "three particles per lane");
     const int64_t NB = p->g.NB, N = p->N;
     deposit_tiles_kernel<ACC><<<(unsigned)(NB * NB * (NB / TILE_B)), 64 * TILE_B, 0, st>>>(
         A, DENSE_MIN, overwrite ? 1 : 0, (ACC *)ysh, (ACC *)zsh, (ACC *)xsh);
@@ -3264,7 +3277,8 @@ static int deposit_impl(lf2mi_parts p, int kind, void *field, int halo, int targ
             unsigned int *dmax = nullptr, hmax[4] = {0u, 0u, 0u, 0u};
             LF2MI_HIP(hipMallocAsync((void **)&dmax, 4 * sizeof(unsigned int), st));
             fill_int_kernel<<<1, 64, 0, st>>>(reinterpret_cast<int *>(dmax), 4, 0);
-            dep_max_kernel<<<grid1(p->np), 256, 0, st>>>(p->np, A.s, A.mass_nv, dmax);
+            dep_max_kernel<<<(unsigned)std::min<int64_t>(grid1(p->np), DMAX_BLOCKS), 256, 0, st>>>(p->np, A.s, A.mass_nv,
+                                                                                          dmax);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) e = hipMemcpyAsync(hmax, dmax, sizeof hmax, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
