@@ -36,6 +36,31 @@ def log(msg):
 GKD_DIRECT_MAX = 192  # bricks below this count go to the gather's chunk kernel
 BRICK = 8
 
+# HBM traffic per launch of this workload's kernels, MiB per dispatch: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE over `bench.py --steps 2 --warmup 1 --no-fp64 --no-cpu --calib`,
+# each scaled by what the counter reads for the run's 1 GiB device copy: FETCH_SIZE 256.1 MiB (factor 1024 / 256.1), WRITE_SIZE 1024.0 MiB
+# (factor 1).  Registry name -> (FETCH_SIZE, WRITE_SIZE) summed over the kernels it times.
+TRAFFIC_MIB = {
+    "rebucket_scatter": (1803.7, 3761.8),
+    "gkd_f32": (1818.8, 3011.8),
+    "gkd_f32_chunks": (1022.2, 668.7),
+    "deposit_f32_scalar": (619.1 + 920.7 + 0.5 + 0.1, 685.2 + 247.6 + 0.2 + 14.2),
+    "fft_x_r2c": (266.8, 520.0),
+    "fft_x_c2r": (272.0, 515.0),
+    "fft_strided_fwd": (272.1, 520.0),
+    "fft_strided_inv": (272.1, 520.0),
+    "fft_y_green": (272.3, 586.0),
+}
+FETCH_FACTOR, WRITE_FACTOR = 1024.0 / 256.1, 1.0
+
+
+def measured_traffic(name: str) -> float | None:
+    """HBM bytes per launch of the kernels behind registry name `name`, or None"""
+    if name not in TRAFFIC_MIB:
+        return None
+    f, w = TRAFFIC_MIB[name]
+    return (f * FETCH_FACTOR + w * WRITE_FACTOR) * 2 ** 20
+
 
 def algorithmic_bytes(name: str, N: int, Np: int, esz: int, np_sparse: int = 0) -> float | None:
     """compulsory HBM bytes of one launch of `name` (reads + writes of its data); the
@@ -205,7 +230,7 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(ab / (avg * 1e-3) / 1e9, 1) if ab else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ab / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ab else None,
-            "traffic": None, "avg_launch_ms": round(avg, 4), "algorithmic_bytes_per_launch": ab}
+            "traffic": measured_traffic(dom), "avg_launch_ms": round(avg, 4), "algorithmic_bytes_per_launch": ab}
     for k, v in kernels.items():
         b = algorithmic_bytes(k, N, Np, 4, np_sparse)
         v["GBps"] = round(b / (v["avg_ms"] * 1e-3) / 1e9, 1) if b else None
