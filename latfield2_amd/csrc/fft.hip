@@ -670,11 +670,18 @@ static int launch_x_tile(const XArgs<T> &A, int S, hipStream_t st) {
     const unsigned grid = (unsigned)((npairs + P - 1) / P);
     const size_t shm = sizeof(cplx<T>) * N * (L + 1);
     LF2MI_PROF(S < 0 ? "fft_x_r2c" : "fft_x_c2r", st);
+    // the component counts of the PM step (1) and of vector (3) and symmetric tensor (6) fields
+    // are compile-time constants, so that the tile's line -> (pair, component) splits are
+    // multiplications
     if (S < 0) {
         if (A.C == 1) fft_x_r2c_kernel<T, N, L, WG, 1><<<grid, WG, shm, st>>>(A);
+        else if (A.C == 3 && L >= 3) fft_x_r2c_kernel<T, N, L, WG, (L >= 3 ? 3 : 0)><<<grid, WG, shm, st>>>(A);
+        else if (A.C == 6 && L >= 6) fft_x_r2c_kernel<T, N, L, WG, (L >= 6 ? 6 : 0)><<<grid, WG, shm, st>>>(A);
         else fft_x_r2c_kernel<T, N, L, WG, 0><<<grid, WG, shm, st>>>(A);
     } else {
         if (A.C == 1) fft_x_c2r_kernel<T, N, L, WG, 1><<<grid, WG, shm, st>>>(A);
+        else if (A.C == 3 && L >= 3) fft_x_c2r_kernel<T, N, L, WG, (L >= 3 ? 3 : 0)><<<grid, WG, shm, st>>>(A);
+        else if (A.C == 6 && L >= 6) fft_x_c2r_kernel<T, N, L, WG, (L >= 6 ? 6 : 0)><<<grid, WG, shm, st>>>(A);
         else fft_x_c2r_kernel<T, N, L, WG, 0><<<grid, WG, shm, st>>>(A);
     }
     LF2MI_LAUNCHED();
