@@ -617,6 +617,8 @@ struct DepArgs {
                              // fixed point counts dimensionless weights)
     int h;
     int tgt;
+    const int *dlist;        // deposit_f32_kernel: a list of items (with its count at dcount)
+    const int *dcount;       // instead of one item per workgroup
 };
 
 template <typename ACC> struct AccOps;
@@ -788,7 +790,7 @@ __device__ __forceinline__ double window_scale(int n, const DepMax &m) {
 // without a mass plane when the mass is global (PM false), with 16-bit cell keys, and the
 // cell counts turn into starts in place.
 template <int KIND, typename ACC, int BC, bool PM>
-__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
+__device__ __forceinline__ void deposit_item(const DepArgs &A, int item) {
     constexpr int C = KindC<KIND>::C, CW = KindC<KIND>::CW;
     constexpr int LO = KindC<KIND>::LO;
     constexpr int NWMAX = MAXB + 1 - LO;
@@ -813,7 +815,7 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
     unsigned short *skey = reinterpret_cast<unsigned short *>(pool + NPL * PL_B);
     unsigned long long *nodes = reinterpret_cast<unsigned long long *>(pool);
     ItemRange r;
-    if (!item_range(blockIdx.x, A.start, A.istart, A.item_bucket, A.g, r)) return;
+    if (!item_range(item, A.start, A.istart, A.item_bucket, A.g, r)) return;
     const int p0 = r.p0, n = r.p1 - r.p0;
     const int B = BC ? BC : A.g.B, N = A.g.N;
     const int NC = B * B * B, NWE = B + 1 - LO;
@@ -993,6 +995,20 @@ __global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
         flush(3);
     }
 #undef LF2MI_DEP_COMP
+}
+
+// one item per workgroup, or (A.dlist) the items of a work list, grid-stride
+template <int KIND, typename ACC, int BC, bool PM>
+__global__ __launch_bounds__(DEP_WG) void deposit_f32_kernel(DepArgs A) {
+    if (A.dlist) {
+        const int cnt = *A.dcount;
+        for (int it = blockIdx.x; it < cnt; it += gridDim.x) {
+            deposit_item<KIND, ACC, BC, PM>(A, A.dlist[it]);
+            __syncthreads();  // the LDS is reused
+        }
+    } else {
+        deposit_item<KIND, ACC, BC, PM>(A, blockIdx.x);
+    }
 }
 
 // Scalar deposit with a global mass: a workgroup takes up to DEP_MERGE consecutive items of
@@ -1395,6 +1411,43 @@ __global__ __launch_bounds__(DG_WG) void dense_groups_kernel(const int *__restri
     for (int k = 0; k < g; ++k) groups[base + k] = make_int2((int)b, k);
 }
 
+// work list of the item kernel: the items of the bricks with dense_min or more particles
+// (one atomic per workgroup of DG_WG threads)
+__global__ __launch_bounds__(DG_WG) void dense_items_kernel(const int *__restrict__ start,
+                                                            const int *__restrict__ istart, int64_t nbk,
+                                                            int dense_min, int *__restrict__ items,
+                                                            int *__restrict__ nitems) {
+    __shared__ int wsum[DG_WG / 64];
+    __shared__ int wbase;
+    const int64_t b = (int64_t)blockIdx.x * DG_WG + threadIdx.x;
+    int g = 0, i0 = 0;
+    if (b < nbk && start[b + 1] - start[b] >= dense_min) {
+        i0 = istart[b];
+        g = istart[b + 1] - i0;
+    }
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int incl = g;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < DG_WG / 64; ++w) {
+            const int v = wsum[w];
+            wsum[w] = tot;
+            tot += v;
+        }
+        wbase = tot > 0 ? atomicAdd(nitems, tot) : 0;
+    }
+    __syncthreads();
+    const int base = wbase + wsum[wave] + incl - g;
+    for (int k = 0; k < g; ++k) items[base + k] = i0 + k;
+}
+
 static constexpr int DENSE_PER = 4;  // particles per thread and round
 
 // persistent: workgroup w takes entries w, w + grid, ... of the work list (full groups are
@@ -1495,6 +1548,485 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_fold_kernel(DepArgs A, c
         return;
     }
     dense_body<ACC>(A, groups, ngroups, (int)blockIdx.x - fold_blocks, (int)gridDim.x - fold_blocks);
+}
+
+// ---------------------------------------------------------------- multi-component tiles
+// The vector, VecVec and CIC-NGP kinds, and the scalar kind with per-particle masses, into
+// a periodic target: the scheme of deposit_tiles_kernel with the components in passes.
+// One wave per brick, TC_TB bricks along x per workgroup.  The brick's first 64 TC_P
+// particles stay in registers across the passes (the rest are read again in each pass),
+// and each pass
+//   1. sums one component into the wave's LDS window (64-bit fixed point with integer LDS
+//      atomics, the scale from the tile's maxima of |q| and |v|, as in deposit_tiles_kernel);
+//   2. adds the x halo planes of the windows into the neighbouring waves' windows;
+//   3. keeps the bricks' own nodes of that component in registers (each thread owns a
+//      column of 8 nodes along z), which are stored (set) or added (without atomics) with
+//      all components of a site together after the last pass, so that every line of the
+//      interleaved field is written once;
+//   4. writes the window nodes outside the bricks in y or z (shell rows of the tile's
+//      consecutive x) and the tile's outer x halo planes (seams) to a buffer, which
+//      fold_rows_kernel and fold_edges_kernel then add to the nodes they land on.
+// A component's window spans nodes [LO, HI] per axis relative to the brick.  On a link axis
+// the particle's two nonzero link weights sit on nodes x - 1, x or on x, x + 1
+// (projections.hpp:1058-1071, whose third weight is zero), so LO = -1 and HI = 8; a vertex
+// axis spans [0, 8], a nearest-grid-point axis [0, 7].
+// Bricks with dense_min or more particles are left out (their windows stay zero) and are
+// deposited afterwards by the item kernel over a work list of their items
+// (dense_items_kernel, deposit_f32_kernel with A.dlist).
+// (measured at 1024^3 with about 512 particles per brick: 4 bricks per tile and 2 particles
+// per lane in registers, at most 128 VGPRs, against 8 bricks per tile or 8 particles per
+// lane: within 5 %, the bricks' own nodes written once per line being what counted)
+static constexpr int TC_P = 2;       // particles per lane kept in registers across the passes
+static constexpr int TC_WMAX = 900;  // largest window: two link axes, 10 x 10 x 9 nodes
+static constexpr int TC_TB = 4;      // bricks per tile (one wave each)
+
+// halo layers below and above the brick along axis d, and the window edge
+__host__ __device__ constexpr int tc_nl(int mask, int d) { return (mask >> d) & 1; }
+__host__ __device__ constexpr int tc_nh(int ngp, int d) { return ((ngp >> d) & 1) ? 0 : 1; }
+__host__ __device__ constexpr int tc_w(int mask, int ngp, int d) { return 8 + tc_nl(mask, d) + tc_nh(ngp, d); }
+// shell rows of a component per brick row: the y halo layers over the window's z range,
+// then the z halo layers over y = 0 .. 7
+__host__ __device__ constexpr int tc_rows(int mask, int ngp) {
+    return (tc_nl(mask, 1) + tc_nh(ngp, 1)) * tc_w(mask, ngp, 2) + (tc_nl(mask, 2) + tc_nh(ngp, 2)) * 8;
+}
+// seam values per tile: its x halo planes (y and z over the window)
+__host__ __device__ constexpr int tc_seam(int mask, int ngp) {
+    return (tc_nl(mask, 0) + tc_nh(ngp, 0)) * tc_w(mask, ngp, 1) * tc_w(mask, ngp, 2);
+}
+// Comp<KIND, c>::MASK and ::NGP as tables for the host
+static constexpr int TC_MASK[5][6] = {{0}, {1, 2, 4}, {0, 3, 5, 0, 6, 0}, {0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+static constexpr int TC_NGP[5][6] = {{0}, {0, 0, 0}, {0, 0, 0, 0, 0, 0}, {1, 2, 4}, {0, 3, 5, 0, 6, 0}};
+template <int KIND, int COMP> constexpr bool tc_table_matches() {
+    return TC_MASK[KIND][COMP] == Comp<KIND, COMP>::MASK && TC_NGP[KIND][COMP] == Comp<KIND, COMP>::NGP;
+}
+static_assert(tc_table_matches<0, 0>() && tc_table_matches<1, 0>() && tc_table_matches<1, 1>() &&
+                  tc_table_matches<1, 2>() && tc_table_matches<2, 0>() && tc_table_matches<2, 1>() &&
+                  tc_table_matches<2, 2>() && tc_table_matches<2, 3>() && tc_table_matches<2, 4>() &&
+                  tc_table_matches<2, 5>() && tc_table_matches<3, 0>() && tc_table_matches<3, 1>() &&
+                  tc_table_matches<3, 2>() && tc_table_matches<4, 0>() && tc_table_matches<4, 1>() &&
+                  tc_table_matches<4, 2>() && tc_table_matches<4, 3>() && tc_table_matches<4, 4>() &&
+                  tc_table_matches<4, 5>(),
+              "host tables of the component masks");
+
+struct TcArgs {
+    int64_t shell_off[6];  // first value of component c's shell rows in the buffer
+    int64_t seam_off[6];   // first value of its seams
+};
+
+// the particles a lane keeps: cell in the brick (x + 8 (y + 8 z), -1 for none), offsets,
+// velocity (vector kinds) and q = m N^3
+struct TcRegs {
+    static constexpr int P = TC_P > 0 ? TC_P : 1;
+    int lc[P];
+    float u[3][P], v[3][P], q[P];
+};
+
+template <int KIND, bool PM>
+__device__ __forceinline__ void tc_load(const DepArgs &A, int p, int &lc, float &u0, float &u1, float &u2, float &v0,
+                                        float &v1, float &v2, float &q) {
+    int x, y, z;
+    decode_cell(A.s.cell[p], A.g, x, y, z);
+    lc = (x & 7) | ((y & 7) << 3) | ((z & 7) << 6);
+    u0 = A.s.off[0][p];
+    u1 = A.s.off[1][p];
+    u2 = A.s.off[2][p];
+    q = PM ? A.s.mass[p] * A.mass_nv : A.mass_nv;
+    if constexpr (KIND > 0) {
+        v0 = A.s.vel[0][p];
+        v1 = A.s.vel[1][p];
+        v2 = A.s.vel[2][p];
+    } else {
+        v0 = v1 = v2 = 0.0f;
+    }
+}
+
+// weights of one axis on nodes b, b + 1 (b moves to x - 1 below a link's midpoint)
+template <int MASK, int NGP, int D>
+__device__ __forceinline__ void tc_weights(float u, int &b, float (&wt)[2]) {
+    if constexpr ((MASK >> D) & 1) {
+        const float t = u - 0.5f;
+        if (t < 0.0f) {
+            const float a = -t;
+            b -= 1;
+            wt[0] = a;
+            wt[1] = 1.0f - a;
+        } else {
+            wt[0] = 1.0f - t;
+            wt[1] = t;
+        }
+    } else if constexpr ((NGP >> D) & 1) {
+        wt[0] = 1.0f;
+        wt[1] = 0.0f;
+    } else {
+        wt[0] = 1.0f - u;
+        wt[1] = u;
+    }
+}
+
+// the window scale of a component: 2^(51 - e) with 2 n max|q| max|v_a| max|v_b| < 2^e, so
+// that a node's sum, with the x halo of the neighbouring brick added, stays below 2^52 and
+// each term below 2^51 (the FMA conversion of dep_term)
+template <int KIND, int COMP>
+__device__ __forceinline__ double tc_scale(int n, float mq, const float (&mv)[3]) {
+    constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
+    double bound = 2.0 * (double)n * (double)mq;
+    if constexpr (VA >= 0) bound *= (double)mv[VA];
+    if constexpr (VB >= 0) bound *= (double)mv[VB];
+    if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
+    int e;
+    frexp(bound, &e);
+    return ldexp(1.0, 51 - e);
+}
+
+template <int KIND, int COMP, bool PM, int TB>
+__device__ __forceinline__ void tc_pass(const DepArgs &A, const TcArgs &T, float *__restrict__ shell,
+                                        unsigned long long (*win)[TC_WMAX], const TcRegs &R, int a, int n, double sc,
+                                        float (&own)[KindC<KIND>::C][8], int tile, int row, int ntile) {
+    constexpr int MASK = Comp<KIND, COMP>::MASK, NGP = Comp<KIND, COMP>::NGP;
+    constexpr int VA = Comp<KIND, COMP>::VA, VB = Comp<KIND, COMP>::VB;
+    constexpr int TX = 8 * TB;  // nodes per tile row
+    constexpr int LX = -tc_nl(MASK, 0), LY = -tc_nl(MASK, 1), LZ = -tc_nl(MASK, 2);
+    constexpr int WX = tc_w(MASK, NGP, 0), WY = tc_w(MASK, NGP, 1), WZ = tc_w(MASK, NGP, 2);
+    constexpr int W3 = WX * WY * WZ, PL = WY * WZ;
+    constexpr int SX = tc_nh(NGP, 0) ? 2 : 1, SY = tc_nh(NGP, 1) ? 2 : 1, SZ = tc_nh(NGP, 2) ? 2 : 1;
+    static_assert(W3 <= TC_WMAX, "the window fits");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int N = A.g.N;
+    unsigned long long *w = win[wave];
+    __syncthreads();  // the previous pass is done reading every window
+    for (int i = lane; i < W3; i += 64) w[i] = 0ull;
+    auto add = [&](int c, float x, float y, float z, float q, float v0, float v1, float v2) {
+        float qc = q;
+        if constexpr (VA >= 0) qc *= VA == 0 ? v0 : (VA == 1 ? v1 : v2);
+        if constexpr (VB >= 0) qc *= VB == 0 ? v0 : (VB == 1 ? v1 : v2);
+        const double qsc = (double)qc * sc;
+        int bx = c & 7, byy = (c >> 3) & 7, bzz = c >> 6;
+        float wx[2], wy[2], wz[2];
+        tc_weights<MASK, NGP, 0>(x, bx, wx);
+        tc_weights<MASK, NGP, 1>(y, byy, wy);
+        tc_weights<MASK, NGP, 2>(z, bzz, wz);
+        const int base = (bx - LX) + WX * ((byy - LY) + WY * (bzz - LZ));
+#pragma unroll
+        for (int kz = 0; kz < SZ; ++kz)
+#pragma unroll
+            for (int ky = 0; ky < SY; ++ky)
+#pragma unroll
+                for (int kx = 0; kx < SX; ++kx)
+                    atomicAdd(&w[base + kx + WX * (ky + WY * kz)], dep_term<float>(qc, wx[kx] * wy[ky] * wz[kz], qsc, 0.0));
+    };
+#pragma unroll
+    for (int k = 0; k < TC_P; ++k)
+        if (R.lc[k] >= 0) add(R.lc[k], R.u[0][k], R.u[1][k], R.u[2][k], R.q[k], R.v[0][k], R.v[1][k], R.v[2][k]);
+    // the particles past the registers, two per lane and round
+    for (int j0 = 64 * TC_P; j0 < n; j0 += 64 * 2) {
+        int c[2];
+        float u0[2], u1[2], u2[2], v0[2], v1[2], v2[2], q[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = j0 + lane + 64 * k;
+            c[k] = -1;
+            if (j < n) tc_load<KIND, PM>(A, a + j, c[k], u0[k], u1[k], u2[k], v0[k], v1[k], v2[k], q[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (c[k] >= 0) add(c[k], u0[k], u1[k], u2[k], q[k], v0[k], v1[k], v2[k]);
+    }
+    __syncthreads();
+    // x halo planes between the tile's waves: the previous brick's x = 8 plane joins this
+    // brick's x = 0 plane, the next brick's x = -1 plane this brick's x = 7 plane (each wave
+    // writes only its own x = 0 and x = 7 planes and reads only halo planes)
+    if constexpr (tc_nh(NGP, 0))
+        if (wave > 0)
+            for (int yz = lane; yz < PL; yz += 64) w[(0 - LX) + WX * yz] += win[wave - 1][(8 - LX) + WX * yz];
+    if constexpr (tc_nl(MASK, 0))
+        if (wave < TB - 1)
+            for (int yz = lane; yz < PL; yz += 64) w[(7 - LX) + WX * yz] += win[wave + 1][(-1 - LX) + WX * yz];
+    __syncthreads();
+    const double inv = 1.0 / sc;
+    auto cvt = [&](unsigned long long v) -> float { return (float)((double)(long long)v * inv); };
+    const int X0 = tile * TX;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;  // this thread's x in the tile, and y
+    // own nodes (x = tx, y = ty, z = 0 .. 7): kept until every component is done
+    {
+        const unsigned long long *src = &win[tx >> 3][((tx & 7) - LX) + WX * (ty - LY)];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) own[COMP][z] = cvt(src[WX * WY * (z - LZ)]);
+    }
+    // shell rows: y halo layers (z over the window), then z halo layers (y = 0 .. 7)
+    constexpr int RY = (tc_nl(MASK, 1) + tc_nh(NGP, 1)) * WZ, RR = tc_rows(MASK, NGP);
+    float *sh = shell + T.shell_off[COMP] + (int64_t)row * RR * N + X0 + tx;
+    for (int r = ty; r < RR; r += 8) {
+        int y, z;
+        if (r < RY) {
+            const int hy = r / WZ;
+            z = LZ + (r - hy * WZ);
+            y = (tc_nl(MASK, 1) && hy == 0) ? -1 : 8;
+        } else {
+            const int s = r - RY, hz = s >> 3;
+            y = s & 7;
+            z = (tc_nl(MASK, 2) && hz == 0) ? -1 : 8;
+        }
+        sh[(int64_t)r * N] = cvt(win[tx >> 3][((tx & 7) - LX) + WX * ((y - LY) + WY * (z - LZ))]);
+    }
+    // seams: the x = -1 plane of the tile's first brick, the x = 8 plane of its last
+    constexpr int NS = tc_nl(MASK, 0) + tc_nh(NGP, 0);
+    if constexpr (NS > 0) {
+        float *se = shell + T.seam_off[COMP] + ((int64_t)row * ntile + tile) * (NS * PL);
+        for (int i = threadIdx.x; i < NS * PL; i += 64 * TB) {
+            const int side = i / PL, yz = i - side * PL;
+            const bool lower = tc_nl(MASK, 0) && side == 0;
+            se[i] = cvt(lower ? win[0][(-1 - LX) + WX * yz] : win[TB - 1][(8 - LX) + WX * yz]);
+        }
+    }
+}
+
+// grid: one workgroup per tile (TB bricks along x) of every row of bricks
+template <int KIND, bool PM, int TB>
+__global__ __launch_bounds__(64 * TB, 4) void deposit_tiles_c_kernel(DepArgs A, int dense_min, int set, TcArgs T,
+                                                                  float *__restrict__ shell) {
+    constexpr int C = KindC<KIND>::C, TX = 8 * TB;
+    __shared__ unsigned long long win[TB][TC_WMAX];
+    __shared__ float smax[TB][4];
+    __shared__ int sn[TB];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int NB = A.g.NB, N = A.g.N, G = N + 2 * A.h, ntile = NB / TB;
+    const int tile = blockIdx.x % ntile, row = blockIdx.x / ntile;
+    const int64_t brick = (int64_t)row * NB + tile * TB + wave;  // bricks are numbered x fastest
+    const int a = A.start[brick];
+    int n = A.start[brick + 1] - a;
+    if (n >= dense_min) n = 0;  // the item kernel's
+    TcRegs R;
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < TC_P; ++k) {
+        const int j = lane + 64 * k;
+        R.lc[k] = -1;
+        if (j < n) {
+            tc_load<KIND, PM>(A, a + j, R.lc[k], R.u[0][k], R.u[1][k], R.u[2][k], R.v[0][k], R.v[1][k], R.v[2][k],
+                              R.q[k]);
+            m[0] = fmaxf(m[0], fabsf(R.q[k]));
+            m[1] = fmaxf(m[1], fabsf(R.v[0][k]));
+            m[2] = fmaxf(m[2], fabsf(R.v[1][k]));
+            m[3] = fmaxf(m[3], fabsf(R.v[2][k]));
+        }
+    }
+    for (int j = 64 * TC_P + lane; j < n; j += 64) {
+        int c;
+        float u0, u1, u2, v0, v1, v2, q;
+        tc_load<KIND, PM>(A, a + j, c, u0, u1, u2, v0, v1, v2, q);
+        m[0] = fmaxf(m[0], fabsf(q));
+        m[1] = fmaxf(m[1], fabsf(v0));
+        m[2] = fmaxf(m[2], fabsf(v1));
+        m[3] = fmaxf(m[3], fabsf(v2));
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[k] = fmaxf(m[k], __shfl_xor(m[k], o));
+    if (lane == 0) {
+        smax[wave][0] = m[0];
+        smax[wave][1] = m[1];
+        smax[wave][2] = m[2];
+        smax[wave][3] = m[3];
+        sn[wave] = n;
+    }
+    __syncthreads();
+    float mq = 0.0f, mv[3] = {0.0f, 0.0f, 0.0f};
+    int nm = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b) {
+        mq = fmaxf(mq, smax[b][0]);
+        mv[0] = fmaxf(mv[0], smax[b][1]);
+        mv[1] = fmaxf(mv[1], smax[b][2]);
+        mv[2] = fmaxf(mv[2], smax[b][3]);
+        nm = max(nm, sn[b]);
+    }
+    float own[C][8];
+#define LF2MI_TC_PASS(I) \
+    tc_pass<KIND, I, PM, TB>(A, T, shell, win, R, a, n, tc_scale<KIND, I>(nm, mq, mv), own, tile, row, ntile);
+    LF2MI_TC_PASS(0)
+    if constexpr (C > 1) {
+        LF2MI_TC_PASS(1)
+        LF2MI_TC_PASS(2)
+    }
+    if constexpr (C > 3) {
+        LF2MI_TC_PASS(3)
+        LF2MI_TC_PASS(4)
+        LF2MI_TC_PASS(5)
+    }
+#undef LF2MI_TC_PASS
+    // the own nodes, all components of a site together (whole lines of the interleaved field)
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int by = row % NB, bz = row / NB;
+    const int64_t i0 = site_index(N, A.h, tile * TX + tx, by * 8 + ty, bz * 8);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        float *f = A.field + (i0 + (int64_t)z * G * G) * C;
+        if (set) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) f[c] = own[c][z];
+        } else {
+            float o[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) o[c] = f[c];
+#pragma unroll
+            for (int c = 0; c < C; ++c) f[c] = o[c] + own[c][z];
+        }
+    }
+}
+
+// What the fold kernels need of each component's shells and seams.  The fold visits each
+// node that receives shell or seam values once, sums every component's contributions and
+// adds them to the field without atomics (nothing else writes the field meanwhile).  A node
+// receives from the windows of neighbouring bricks: in y, a brick row's y = 8 layer lands on
+// the next row's y = 0 layer and its y = -1 layer on the previous row's y = 7 layer (link
+// axes), likewise in z; in x, a tile's outer planes land on the neighbouring tiles' edge
+// nodes.  Receiving layers (union over the components): sy = y layers in the brick (0: a
+// component has an upper y halo, 7: a lower one), cy the other y layers, likewise z, and sx
+// the x positions in the tile (0 and TX - 1).
+struct TcFold {
+    int64_t shell_off[6], seam_off[6];  // as in TcArgs
+    int rows[6], ns[6];                 // shell rows per brick row, seam planes per tile
+    int lo[6][3], hi[6][3], w[6][3];    // window per axis
+    int C, TX, ntile;
+    int ny, nz, nx;                     // receiving layers
+    int sy[2], sz[2], sx[2], cy[8], cz[8];
+};
+
+// the y and z shell rows of component c that land on the row (Y, Z): their offsets in the
+// buffer (the value for x = X at offset + X)
+__device__ __forceinline__ int tc_row_sources(const TcFold &F, int N, int NB, int c, int Y, int Z, int64_t (&off)[8]) {
+    const int by = Y >> 3, ly = Y & 7, bz = Z >> 3, lz = Z & 7;
+    const int LY = F.lo[c][1], HY = F.hi[c][1], LZ = F.lo[c][2], HZ = F.hi[c][2];
+    const int WZ = F.w[c][2], RR = F.rows[c], RY = (F.w[c][1] - 8) * WZ;
+    int n = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zp = lz - 8 * dz;
+        if (zp < LZ || zp > HZ) continue;
+        const int bzs = dz < 0 ? (bz == 0 ? NB - 1 : bz - 1) : (dz > 0 ? (bz + 1 == NB ? 0 : bz + 1) : bz);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yp = ly - 8 * dy;
+            if (yp < LY || yp > HY || (dy == 0 && dz == 0)) continue;
+            const int bys = dy < 0 ? (by == 0 ? NB - 1 : by - 1) : (dy > 0 ? (by + 1 == NB ? 0 : by + 1) : by);
+            const int64_t srow = bys + (int64_t)NB * bzs;
+            const int r = dy != 0 ? (yp == 8 ? -LY : 0) * WZ + (zp - LZ) : RY + (zp == 8 ? -LZ : 0) * 8 + yp;
+            off[n++] = F.shell_off[c] + (srow * RR + r) * N;
+        }
+    }
+    return n;
+}
+
+// the seam values of component c that land on node (X, Y, Z): from the previous tile's upper
+// plane on the tile's first x, from the next tile's lower plane on its last x
+__device__ __forceinline__ float tc_seam_sum(const TcFold &F, const float *__restrict__ shell, int NB, int c, int X,
+                                             int Y, int Z) {
+    const int tile = X / F.TX, lxt = X - tile * F.TX;
+    int st, side;
+    if (lxt == 0 && F.hi[c][0] == 8) {
+        st = tile == 0 ? F.ntile - 1 : tile - 1;
+        side = -F.lo[c][0];
+    } else if (lxt == F.TX - 1 && F.lo[c][0] == -1) {
+        st = tile + 1 == F.ntile ? 0 : tile + 1;
+        side = 0;
+    } else {
+        return 0.0f;
+    }
+    const int by = Y >> 3, ly = Y & 7, bz = Z >> 3, lz = Z & 7;
+    const int LY = F.lo[c][1], HY = F.hi[c][1], LZ = F.lo[c][2], HZ = F.hi[c][2], WY = F.w[c][1];
+    const int PL = WY * F.w[c][2];
+    float s = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zp = lz - 8 * dz;
+        if (zp < LZ || zp > HZ) continue;
+        const int bzs = dz < 0 ? (bz == 0 ? NB - 1 : bz - 1) : (dz > 0 ? (bz + 1 == NB ? 0 : bz + 1) : bz);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yp = ly - 8 * dy;
+            if (yp < LY || yp > HY) continue;
+            const int bys = dy < 0 ? (by == 0 ? NB - 1 : by - 1) : (dy > 0 ? (by + 1 == NB ? 0 : by + 1) : by);
+            const int64_t srow = bys + (int64_t)NB * bzs;
+            s += shell[F.seam_off[c] + ((srow * F.ntile + st) * F.ns[c] + side) * PL + (yp - LY) + WY * (zp - LZ)];
+        }
+    }
+    return s;
+}
+
+// the contributions of every component to node (X, Y, Z), added to the field
+__device__ __forceinline__ void tc_fold_node(float *__restrict__ field, int N, int h, int NB, const TcFold &F,
+                                             const float *__restrict__ shell, int X, int Y, int Z) {
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    bool any = false;
+    for (int c = 0; c < F.C; ++c) {
+        int64_t off[8];
+        const int n = tc_row_sources(F, N, NB, c, Y, Z, off);
+        float s = tc_seam_sum(F, shell, NB, c, X, Y, Z);
+        for (int k = 0; k < n; ++k) s += shell[off[k] + X];
+        v[c] = s;
+        any |= s != 0.0f;
+    }
+    if (!any) return;
+    float *f = field + site_index(N, h, X, Y, Z) * F.C;
+    for (int c = 0; c < F.C; ++c) f[c] += v[c];
+}
+
+// the receiving nodes of whole rows along x: the sy layers (every z), then the cy layers
+// with the sz layers; one workgroup per row (its shell sources found once), x across the
+// threads
+__global__ __launch_bounds__(256) void fold_rows_kernel(float *__restrict__ field, int N, int h, int NB, TcFold F,
+                                                        const float *__restrict__ shell) {
+    __shared__ int64_t soff[6][8];
+    __shared__ int scnt[6];
+    int64_t b = blockIdx.x;
+    const int64_t rowsY = (int64_t)F.ny * NB * N;
+    int Y, Z;
+    if (b < rowsY) {
+        const int yi = (int)(b % (F.ny * NB));
+        Z = (int)(b / (F.ny * NB));
+        Y = (yi / F.ny) * 8 + F.sy[yi % F.ny];
+    } else {
+        b -= rowsY;
+        const int my = (8 - F.ny) * NB;
+        const int yi = (int)(b % my), zi = (int)(b / my);
+        Y = (yi / (8 - F.ny)) * 8 + F.cy[yi % (8 - F.ny)];
+        Z = (zi / F.nz) * 8 + F.sz[zi % F.nz];
+    }
+    if ((int)threadIdx.x < F.C) {
+        int64_t off[8];
+        const int c = threadIdx.x, n = tc_row_sources(F, N, NB, c, Y, Z, off);
+        for (int k = 0; k < n; ++k) soff[c][k] = off[k];
+        scnt[c] = n;
+    }
+    __syncthreads();
+    const int64_t f0 = site_index(N, h, 0, Y, Z) * F.C;
+    for (int X = threadIdx.x; X < N; X += 256) {
+        float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        bool any = false;
+        const int lxt = X % F.TX;
+        const bool edge = lxt == 0 || lxt == F.TX - 1;
+        for (int c = 0; c < F.C; ++c) {
+            float s = edge ? tc_seam_sum(F, shell, NB, c, X, Y, Z) : 0.0f;
+            for (int k = 0; k < scnt[c]; ++k) s += shell[soff[c][k] + X];
+            v[c] = s;
+            any |= s != 0.0f;
+        }
+        if (!any) continue;
+        float *f = field + f0 + (int64_t)X * F.C;
+        for (int c = 0; c < F.C; ++c) f[c] += v[c];
+    }
+}
+
+// the remaining receiving nodes: tile edges (sx) in the cy layers and cz layers
+__global__ __launch_bounds__(256) void fold_edges_kernel(float *__restrict__ field, int N, int h, int NB, TcFold F,
+                                                         const float *__restrict__ shell, int64_t total) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const int nxt = F.nx * F.ntile, my = (8 - F.ny) * NB;
+    const int xi = (int)(t % nxt);
+    const int64_t r = t / nxt;
+    const int yi = (int)(r % my), zi = (int)(r / my);
+    const int X = (xi / F.nx) * F.TX + F.sx[xi % F.nx];
+    const int Y = (yi / (8 - F.ny)) * 8 + F.cy[yi % (8 - F.ny)];
+    const int Z = (zi / (8 - F.nz)) * 8 + F.cz[zi % (8 - F.nz)];
+    tc_fold_node(field, N, h, NB, F, shell, X, Y, Z);
 }
 
 // maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
@@ -2339,6 +2871,8 @@ struct lf2mi_parts_s {
     int *nchunk;   // [number of chunks, claim counter]
     int2 *dgroups; // work list of the dense deposit kernel (at most cap / group + bricks)
     int *ndgroup;  // [number of entries, unused]
+    int *ditems;   // work list of the item kernel after deposit_tiles_c_kernel (items of dense bricks)
+    int *nditem;   // [number of entries, unused]
     unsigned long long *fx;
     int64_t fx_n;
     void *shell;          // shell and seam buffers of the tiles deposit kernel (grown on demand)
@@ -2481,6 +3015,95 @@ static int launch_tiles_dense(lf2mi_parts_s *p, const DepArgs &A, bool overwrite
     LF2MI_LAUNCHED();
     deposit_dense_fold_kernel<ACC><<<FOLD_BLOCKS + DENSE_WORKERS, DEP_WG, 0, st>>>(
         A, p->dgroups, p->ndgroup, (const ACC *)ysh, (const ACC *)zsh, (const ACC *)xsh, FOLD_BLOCKS);
+    LF2MI_LAUNCHED();
+    return LF2MI_OK;
+}
+
+// the other kinds (and the scalar kind with per-particle masses) into a periodic target
+// with 8^3 bricks
+static bool tiles_c_path(const lf2mi_parts_s *p, int kind, int target, bool pm) {
+    return (kind != LF2MI_PROJ_SCALAR || pm) && target == LF2MI_DEP_PERIODIC && p->g.B == 8 &&
+           p->g.NB % TC_TB == 0 && p->N <= 1024;
+}
+
+// the shell buffer of deposit_tiles_c_kernel (grown on demand), its layout, the fold and the
+// item deposit of the dense bricks
+template <int KIND>
+static int launch_tiles_c(lf2mi_parts_s *p, DepArgs A, bool pm, bool overwrite, hipStream_t st) {
+    constexpr int C = KindC<KIND>::C, TB = TC_TB;
+    const int64_t N = p->N, NB = p->g.NB, ntile = NB / TB;
+    TcArgs T{};
+    TcFold F{};
+    F.C = C;
+    F.TX = 8 * TB;
+    F.ntile = (int)ntile;
+    bool ry[2] = {false, false}, rz[2] = {false, false}, rx[2] = {false, false};  // layers 0 / 7
+    int64_t off = 0;
+    for (int c = 0; c < C; ++c) {
+        const int mask = TC_MASK[KIND][c], ngp = TC_NGP[KIND][c];
+        T.shell_off[c] = F.shell_off[c] = off;
+        F.rows[c] = tc_rows(mask, ngp);
+        F.ns[c] = tc_nl(mask, 0) + tc_nh(ngp, 0);
+        for (int d = 0; d < 3; ++d) {
+            F.lo[c][d] = -tc_nl(mask, d);
+            F.hi[c][d] = 7 + tc_nh(ngp, d);
+            F.w[c][d] = tc_w(mask, ngp, d);
+        }
+        ry[0] |= tc_nh(ngp, 1) != 0;
+        ry[1] |= tc_nl(mask, 1) != 0;
+        rz[0] |= tc_nh(ngp, 2) != 0;
+        rz[1] |= tc_nl(mask, 2) != 0;
+        rx[0] |= tc_nh(ngp, 0) != 0;
+        rx[1] |= tc_nl(mask, 0) != 0;
+        off += NB * NB * F.rows[c] * N;
+    }
+    int64_t seams = 0;
+    for (int c = 0; c < C; ++c) {
+        T.seam_off[c] = F.seam_off[c] = off + seams;
+        seams += NB * NB * ntile * tc_seam(TC_MASK[KIND][c], TC_NGP[KIND][c]);
+    }
+    auto layers = [](const bool (&r)[2], int last, int *s, int *cpl, int &ns) {
+        ns = 0;
+        int nc = 0;
+        for (int l = 0; l <= 7; ++l) {
+            const bool recv = (l == 0 && r[0]) || (l == 7 && r[1]);
+            if (recv) s[ns++] = l == 7 ? last : 0;
+            else if (cpl) cpl[nc++] = l;
+        }
+    };
+    layers(ry, 7, F.sy, F.cy, F.ny);
+    layers(rz, 7, F.sz, F.cz, F.nz);
+    layers(rx, F.TX - 1, F.sx, nullptr, F.nx);
+    const int64_t bytes = (off + seams) * (int64_t)sizeof(float);
+    if (p->shell_bytes < bytes) {
+        if (p->shell) hipFree(p->shell);
+        p->shell = nullptr;
+        p->shell_bytes = 0;
+        LF2MI_HIP(hipMalloc(&p->shell, (size_t)bytes));
+        p->shell_bytes = bytes;
+    }
+    float *shell = (float *)p->shell;
+    const unsigned grid = (unsigned)(NB * NB * ntile);
+    if (pm) deposit_tiles_c_kernel<KIND, true, TB><<<grid, 64 * TB, 0, st>>>(A, DENSE_MIN, overwrite ? 1 : 0, T, shell);
+    else deposit_tiles_c_kernel<KIND, false, TB><<<grid, 64 * TB, 0, st>>>(A, DENSE_MIN, overwrite ? 1 : 0, T, shell);
+    LF2MI_LAUNCHED();
+    const int64_t rows = (int64_t)F.ny * NB * N + (int64_t)(8 - F.ny) * NB * F.nz * NB;
+    if (rows > 0) {
+        fold_rows_kernel<<<(unsigned)rows, 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell);
+        LF2MI_LAUNCHED();
+    }
+    const int64_t edges = (int64_t)F.nx * ntile * (8 - F.ny) * NB * (8 - F.nz) * NB;
+    if (edges > 0) {
+        fold_edges_kernel<<<grid1(edges), 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell, edges);
+        LF2MI_LAUNCHED();
+    }
+    if (int rc = fill_int(p->nditem, 2, 0, st)) return rc;
+    dense_items_kernel<<<grid1(p->g.nbk, DG_WG), DG_WG, 0, st>>>(p->bstart, p->istart, p->g.nbk, DENSE_MIN, p->ditems,
+                                                                 p->nditem);
+    LF2MI_LAUNCHED();
+    A.dlist = p->ditems;
+    A.dcount = p->nditem;
+    launch_deposit<KIND, float>(A, true, (unsigned)std::min<int64_t>(item_grid(p), DENSE_WORKERS), st);
     LF2MI_LAUNCHED();
     return LF2MI_OK;
 }
@@ -2641,6 +3264,8 @@ int lf2mi_parts_create(int N, int64_t capacity, int dtype, int flags, lf2mi_part
         A((void **)&p->nchunk, 2 * sizeof(int));
         A((void **)&p->dgroups, (size_t)(cap / (DEP_MERGE * ITEM_P) + p->g.nbk + 1) * sizeof(int2));
         A((void **)&p->ndgroup, 2 * sizeof(int));
+        A((void **)&p->ditems, (size_t)(p->g.nbk + cap / ITEM_P + 1) * sizeof(int));
+        A((void **)&p->nditem, 2 * sizeof(int));
     } else {
         Soa64 &s = p->b64;
         A((void **)&s.cell, cap * sizeof(int32_t));
@@ -3339,6 +3964,15 @@ static int deposit_impl(lf2mi_parts p, int kind, void *field, int halo, int targ
     LF2MI_PROF(names32[kind], st);
     const bool b8 = p->g.B == 8;
     if (tiles_path(p, kind, target, pm)) return launch_tiles_dense<float>(p, A, overwrite, st);
+    if (tiles_c_path(p, kind, target, pm)) {
+        switch (kind) {
+            case 0: return launch_tiles_c<0>(p, A, pm, overwrite, st);
+            case 1: return launch_tiles_c<1>(p, A, pm, overwrite, st);
+            case 2: return launch_tiles_c<2>(p, A, pm, overwrite, st);
+            case 3: return launch_tiles_c<3>(p, A, pm, overwrite, st);
+            default: return launch_tiles_c<4>(p, A, pm, overwrite, st);
+        }
+    }
     switch (kind) {
         case 0: launch_deposit<0, float>(A, b8, g, st); break;
         case 1: launch_deposit<1, float>(A, b8, g, st); break;

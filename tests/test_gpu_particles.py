@@ -182,10 +182,52 @@ def test_cic_ngp_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mo
             assert rel_l2(got[..., c], ref[..., c]) <= tol, (mode, f.C, c, rel_l2(got[..., c], ref[..., c]))
 
 
+@pytest.mark.parametrize("N,n,clustered,masses", [
+    (64, 360000, False, False),   # ~700 per brick: most of them read again in every component pass
+    (64, 200000, True, True),     # a blob of dense bricks (item kernel) among sparse ones
+    (128, 400000, True, False),   # two tiles per row of bricks: seams between tiles and across the edge
+    (128, 400000, True, True)])
+def test_periodic_tile_deposits_match_oracle(N, n, clustered, masses):
+    """the periodic FP32 deposits on lattices whose rows of bricks split into tiles of 8:
+    every kind (and the scalar kind with per-particle masses) against the FP64 oracle"""
+    announce(f"tile deposits N={N} n={n} clustered={clustered} masses={masses}")
+    pos, vel, mass = _random_particles(n, 11 * N + n, clustered)
+    parts, _ = _upload(N, pos, vel, "f32", mass if masses else None)
+    d = parts.download()
+    m = d["mass"] if masses else None
+    h = 1
+    lat = lf.Lattice(3, N, h)
+    cases = ((lf.PROJ_SCALAR, 1, lambda r: orc.scalar_project(d["pos"], r, N, h, mass=m), orc.scalar_comm),
+             (lf.PROJ_VECTOR, 3, lambda r: orc.vector_project(d["pos"], d["vel"], r, N, h, mass=m), orc.vector_comm),
+             (lf.PROJ_VECVEC, 6, lambda r: orc.vecvec_project(d["pos"], d["vel"], r, N, h, mass=m), orc.vecvec_comm),
+             (lf.PROJ_VECTOR_NGP, 3, lambda r: orc.vector_ngp_project(d["pos"], d["vel"], r, N, h, mass=m),
+              orc.vector_ngp_comm),
+             (lf.PROJ_SYMTENSOR_NGP, 6, lambda r: orc.symtensor_ngp_project(d["pos"], d["vel"], r, N, h, mass=m),
+              orc.symtensor_ngp_comm))
+    s = slice(h, N + h)
+    for kind, C, project, comm in cases:
+        f = lf.Field(lat, C, lf.F32)
+        lf.projection_init(f)
+        parts.deposit(kind, f, lf.DEP_PERIODIC)
+        torch.cuda.synchronize()
+        ref = orc.real_field(N, h, C)
+        project(ref)
+        comm(ref, N, h)
+        G = N + 2 * h
+        got = real_np(f).reshape(G, G, G, C)
+        for c in range(C):
+            err = rel_l2(got[s, s, s, c], ref[s, s, s, c])
+            assert err <= 1e-6, (kind, c, err)
+        # the ghost layer is not written by a periodic deposit
+        ghosts = got.copy()
+        ghosts[s, s, s] = 0.0
+        assert not ghosts.any(), kind
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_deposits_of_an_empty_container_add_nothing(prec):
-    announce(f"empty container deposits {prec}")
-    N = 16
+@pytest.mark.parametrize("N", [16, 64])
+def test_deposits_of_an_empty_container_add_nothing(prec, N):
+    announce(f"empty container deposits {prec} N={N}")
     parts = lf.Particles(N, 10, DT[prec])
     lat = lf.Lattice(3, N, 1)
     for kind, C in ((lf.PROJ_SCALAR, 1), (lf.PROJ_VECTOR, 3), (lf.PROJ_VECVEC, 6), (lf.PROJ_VECTOR_NGP, 3),
