@@ -1969,12 +1969,14 @@ __device__ __forceinline__ void tc_fold_node(float *__restrict__ field, int N, i
 }
 
 // the receiving nodes of whole rows along x: the sy layers (every z), then the cy layers
-// with the sz layers; one workgroup per row (its shell sources found once), x across the
-// threads
+// with the sz layers; one workgroup per row.  The row's shell sources are found once and
+// summed for every x (coalesced); then the row's tile-edge nodes add their seam values, one
+// thread each (after the barrier, which orders the two updates of those nodes).
+template <int C>
 __global__ __launch_bounds__(256) void fold_rows_kernel(float *__restrict__ field, int N, int h, int NB, TcFold F,
                                                         const float *__restrict__ shell) {
-    __shared__ int64_t soff[6][8];
-    __shared__ int scnt[6];
+    __shared__ int64_t soff[C][8];
+    __shared__ int scnt[C];
     int64_t b = blockIdx.x;
     const int64_t rowsY = (int64_t)F.ny * NB * N;
     int Y, Z;
@@ -1989,28 +1991,43 @@ __global__ __launch_bounds__(256) void fold_rows_kernel(float *__restrict__ fiel
         Y = (yi / (8 - F.ny)) * 8 + F.cy[yi % (8 - F.ny)];
         Z = (zi / F.nz) * 8 + F.sz[zi % F.nz];
     }
-    if ((int)threadIdx.x < F.C) {
+    if ((int)threadIdx.x < C) {
         int64_t off[8];
         const int c = threadIdx.x, n = tc_row_sources(F, N, NB, c, Y, Z, off);
         for (int k = 0; k < n; ++k) soff[c][k] = off[k];
         scnt[c] = n;
     }
     __syncthreads();
-    const int64_t f0 = site_index(N, h, 0, Y, Z) * F.C;
+    float *f0 = field + site_index(N, h, 0, Y, Z) * C;
     for (int X = threadIdx.x; X < N; X += 256) {
-        float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float v[C];
         bool any = false;
-        const int lxt = X % F.TX;
-        const bool edge = lxt == 0 || lxt == F.TX - 1;
-        for (int c = 0; c < F.C; ++c) {
-            float s = edge ? tc_seam_sum(F, shell, NB, c, X, Y, Z) : 0.0f;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            float s = 0.0f;
             for (int k = 0; k < scnt[c]; ++k) s += shell[soff[c][k] + X];
             v[c] = s;
             any |= s != 0.0f;
         }
         if (!any) continue;
-        float *f = field + f0 + (int64_t)X * F.C;
-        for (int c = 0; c < F.C; ++c) f[c] += v[c];
+        float *f = f0 + (int64_t)X * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) f[c] += v[c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < F.nx * F.ntile; e += 256) {
+        const int X = (e / F.nx) * F.TX + F.sx[e % F.nx];
+        float v[C];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            v[c] = tc_seam_sum(F, shell, NB, c, X, Y, Z);
+            any |= v[c] != 0.0f;
+        }
+        if (!any) continue;
+        float *f = f0 + (int64_t)X * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) f[c] += v[c];
     }
 }
 
@@ -3089,7 +3106,7 @@ static int launch_tiles_c(lf2mi_parts_s *p, DepArgs A, bool pm, bool overwrite, 
     LF2MI_LAUNCHED();
     const int64_t rows = (int64_t)F.ny * NB * N + (int64_t)(8 - F.ny) * NB * F.nz * NB;
     if (rows > 0) {
-        fold_rows_kernel<<<(unsigned)rows, 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell);
+        fold_rows_kernel<C><<<(unsigned)rows, 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell);
         LF2MI_LAUNCHED();
     }
     const int64_t edges = (int64_t)F.nx * ntile * (8 - F.ny) * NB * (8 - F.nz) * NB;
