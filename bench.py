@@ -168,6 +168,62 @@ def cpu_baseline(lf, torch, N, threads, cores):
             "sample_1core": f"the same {N}^3 step on one thread"}
 
 
+def deposit_kinds(args):
+    """--deposit-kinds: every periodic FP32 deposit kind at N^3 (default 1024^3) with N^3
+    particles on the lattice plus a Gaussian displacement (sigma 1 cell), velocities
+    N(0, 0.01), mass 1/N^3: the large multi-component workload.  One JSON line per kind
+    (the scalar kind with per-particle masses as "scalar_per_particle_mass"): ms per deposit
+    (the mean of --steps deposits after --warmup, HIP events on the launch stream) and the
+    per-kernel split."""
+    import torch
+
+    import latfield2_amd as lf
+    from latfield2_amd import _lib
+
+    torch.cuda.set_device(0)
+    N = args.n
+    names = {lf.PROJ_VECTOR: "vector", lf.PROJ_VECVEC: "VecVec", lf.PROJ_VECTOR_NGP: "vector_CIC_NGP",
+             lf.PROJ_SYMTENSOR_NGP: "symtensor_CIC_NGP", lf.PROJ_SCALAR: "scalar"}
+    comps = {lf.PROJ_SCALAR: 1, lf.PROJ_VECTOR: 3, lf.PROJ_VECVEC: 6, lf.PROJ_VECTOR_NGP: 3, lf.PROJ_SYMTENSOR_NGP: 6}
+    for pm in (False, True):
+        log(f"{N}^3 lattice + sigma 1 cell, per-particle masses: {pm}")
+        parts = lf.Particles(N, N ** 3, lf.F32, mass=1.0 / N ** 3, per_particle_mass=pm)
+        parts.ic_lattice(N, 1.0, 0.01, 5)
+        if pm:
+            parts.mass_from(lf.MASS_PART, -1)
+        lat = lf.Lattice(3, N, 1)
+        kinds = [lf.PROJ_SCALAR] if pm else [lf.PROJ_SCALAR, lf.PROJ_VECTOR, lf.PROJ_VECVEC, lf.PROJ_VECTOR_NGP,
+                                             lf.PROJ_SYMTENSOR_NGP]
+        for kind in kinds:
+            C = comps[kind]
+            f = lf.Field(lat, C, lf.F32)
+            f.zero()
+            for _ in range(args.warmup):
+                parts.deposit(kind, f, lf.DEP_PERIODIC, lf.ACC_NATIVE)
+            torch.cuda.synchronize()
+            _lib.profile_reset()
+            _lib.profile_enable(True)
+            for _ in range(args.steps):
+                parts.deposit(kind, f, lf.DEP_PERIODIC, lf.ACC_NATIVE)
+            torch.cuda.synchronize()
+            _lib.profile_enable(False)
+            pr = _lib.profile_read()
+            ms = sum(t for t, _ in pr.values()) / args.steps
+            # particles read (cell, offsets, velocity for the vector kinds, mass), the
+            # field's interior read and written (the deposit adds)
+            pb = 16 + (12 if kind != lf.PROJ_SCALAR else 0) + (4 if pm else 0)
+            ab = N ** 3 * pb + 2 * N ** 3 * C * 4
+            name = names[kind] + ("_per_particle_mass" if pm else "")
+            print(json.dumps({"deposit": name, "mesh": N, "particles": N ** 3, "components": C,
+                              "ms": round(ms, 3), "algorithmic_GB": round(ab / 1e9, 2),
+                              "GBps": round(ab / (ms * 1e-3) / 1e9, 1),
+                              "stages": {k: round(t / args.steps, 3) for k, (t, c) in pr.items()}}), flush=True)
+            del f
+            torch.cuda.empty_cache()
+        del parts
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,13 +234,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="start with a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--deposit-kinds", action="store_true",
+                    help="instead of the PM step: every deposit kind on the large lattice workload "
+                         "(--n defaults to 1024 here), one JSON line per kind")
     args = ap.parse_args()
+    if args.deposit_kinds and "--n" not in sys.argv:
+        args.n = 1024
     if args.steps < 1 or args.warmup < 0:
         ap.error("--steps must be at least 1 and --warmup at least 0")
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.deposit_kinds:
+        deposit_kinds(args)
+        return
 
     import torch
 

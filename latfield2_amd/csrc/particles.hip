@@ -3155,33 +3155,41 @@ using namespace lf2mi;
 
 namespace lf2mi {
 
-// sum of the stored per-particle masses into *total
+// sum of the stored per-particle masses: per-workgroup partial sums (grid-stride, a tree in
+// LDS), then mass_total_kernel adds the partials in order, so the total is the same from
+// run to run
 template <typename T>
-__global__ void sum_mass_kernel(int64_t np, const T *__restrict__ mass, double *__restrict__ total) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double m = i < np ? (double)mass[i] : 0.0;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
-    if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
+__global__ __launch_bounds__(RED_WG) void mass_partial_kernel(int64_t np, const T *__restrict__ mass,
+                                                              double *__restrict__ partial) {
+    __shared__ double lds[RED_WG];
+    double m = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * RED_WG;
+    for (int64_t i = (int64_t)blockIdx.x * RED_WG + threadIdx.x; i < np; i += stride) m += (double)mass[i];
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = RED_WG / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) lds[threadIdx.x] += lds[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+__global__ void mass_total_kernel(const double *__restrict__ partial, int nblocks, double *__restrict__ total) {
+    double t = 0.0;
+    for (int b = 0; b < nblocks; ++b) t += partial[b];
+    *total = t;
 }
 
 // per-particle masses from the value of type S (double, or float) at byte `off` of each
-// payload; sum into *total
+// payload
 template <typename T, typename S>
 __global__ void payload_mass_kernel(int64_t np, const unsigned long long *__restrict__ pl, int plw, int64_t off,
-                                    T *__restrict__ mass, double *__restrict__ total) {
+                                    T *__restrict__ mass) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0.0;
-    if (i < np) {
-        const unsigned char *b = reinterpret_cast<const unsigned char *>(pl + i * plw);
-        S v;
-        memcpy(&v, b + off, sizeof(S));
-        mass[i] = (T)v;
-        m = (double)(T)v;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o);
-    if (lane_id() == 0 && m != 0.0) atomicAdd(total, m);
+    if (i >= np) return;
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(pl + i * plw);
+    S v;
+    memcpy(&v, b + off, sizeof(S));
+    mass[i] = (T)v;
 }
 
 // the Real[3] at byte `off` of each payload into three planes of `cap` values
@@ -3620,39 +3628,30 @@ int lf2mi_parts_mass_from(lf2mi_parts p, int where, int64_t offset, void *stream
         return fail(LF2MI_EINVAL, "mass offset %lld outside the payload", (long long)offset);
     }
     hipStream_t st = as_stream(stream);
-    double *dsum = nullptr;
-    LF2MI_HIP(hipMalloc((void **)&dsum, sizeof(double)));
-    LF2MI_HIP(hipMemsetAsync(dsum, 0, sizeof(double), st));
-    int rc = LF2MI_OK;
-    if (p->np > 0) {
-        if (offset >= 0) {
-            const bool f4 = where == LF2MI_MASS_PART_F32;
-            const unsigned g = grid1(p->np);
-            if (p->dtype == LF2MI_F32) {
-                Soa32 &s = p->b32[p->cur];
-                if (f4) payload_mass_kernel<float, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
-                else payload_mass_kernel<float, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
-            } else {
-                Soa64 &s = p->b64;
-                if (f4) payload_mass_kernel<double, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
-                else payload_mass_kernel<double, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass, dsum);
-            }
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) rc = hip_fail(e, "payload_mass_kernel", __FILE__, __LINE__);
-        } else if (p->dtype == LF2MI_F32) {
-            sum_mass_kernel<float><<<grid1(p->np), 256, 0, st>>>(p->np, p->b32[p->cur].mass, dsum);
+    double *res = p->red + (int64_t)RED_BLOCKS * RED_Q;  // the partials before it
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, (p->np + RED_WG - 1) / RED_WG));
+    if (offset >= 0 && p->np > 0) {
+        const bool f4 = where == LF2MI_MASS_PART_F32;
+        const unsigned g = grid1(p->np);
+        if (p->dtype == LF2MI_F32) {
+            Soa32 &s = p->b32[p->cur];
+            if (f4) payload_mass_kernel<float, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass);
+            else payload_mass_kernel<float, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass);
         } else {
-            sum_mass_kernel<double><<<grid1(p->np), 256, 0, st>>>(p->np, p->b64.mass, dsum);
+            Soa64 &s = p->b64;
+            if (f4) payload_mass_kernel<double, float><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass);
+            else payload_mass_kernel<double, double><<<g, 256, 0, st>>>(p->np, s.pl, p->plw, offset, s.mass);
         }
+        LF2MI_LAUNCHED();
     }
+    if (p->dtype == LF2MI_F32) mass_partial_kernel<float><<<nb, RED_WG, 0, st>>>(p->np, p->b32[p->cur].mass, p->red);
+    else mass_partial_kernel<double><<<nb, RED_WG, 0, st>>>(p->np, p->b64.mass, p->red);
+    LF2MI_LAUNCHED();
+    mass_total_kernel<<<1, 1, 0, st>>>(p->red, nb, res);
+    LF2MI_LAUNCHED();
     double total = 0.0;
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(&total, dsum, sizeof(double), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = hip_fail(e, "mass sum", __FILE__, __LINE__);
-    }
-    hipFree(dsum);
-    if (rc) return rc;
+    LF2MI_HIP(hipMemcpyAsync(&total, res, sizeof(double), hipMemcpyDeviceToHost, st));
+    LF2MI_HIP(hipStreamSynchronize(st));
     p->mass_src = 1;
     p->total_mass = total;
     return LF2MI_OK;
