@@ -278,7 +278,9 @@ template <typename T> struct XArgs {
 // ---------------------------------------------------------------- strided passes (y, z)
 // grid: x = column tiles of L, y = batches.  MODE 0: one transform of sign S, HBM to HBM.
 // MODE 1: forward transform, Green multiply, inverse transform (fused Poisson y pass).
-template <typename T, int N, int L, int WG, int S, int MODE>
+// I32: every element offset of a tile, (N - 1) * es + L, fits in 32 bits (one integer
+// multiply-add per element instead of a 64-bit one: 4-8 % of the strided passes at 512^3)
+template <typename T, int N, int L, int WG, int S, int MODE, bool I32>
 __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
     using V = cplx<T>;
     extern __shared__ unsigned char smem_raw[];
@@ -299,11 +301,17 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
     const V *in = A.in + A.in_col0 + batch * A.in_bs + col0;
     V *out = A.out + A.out_col0 + batch * A.out_bs + col0;
     const int ncols = A.ncols - col0;
-    auto gload = [in, ncols, &A](int l, int e) {
-        return l < ncols ? in[(int64_t)e * A.in_es + l] : mk<T>(0, 0);
+    const int ies = (int)A.in_es, oes = (int)A.out_es;
+    auto gload = [in, ncols, ies, &A](int l, int e) {
+        if constexpr (I32) return l < ncols ? in[e * ies + l] : mk<T>(0, 0);
+        else return l < ncols ? in[(int64_t)e * A.in_es + l] : mk<T>(0, 0);
     };
-    auto gstore = [out, ncols, &A](int l, int e, V v) {
-        if (l < ncols) out[(int64_t)e * A.out_es + l] = v;
+    auto gstore = [out, ncols, oes, &A](int l, int e, V v) {
+        if constexpr (I32) {
+            if (l < ncols) out[e * oes + l] = v;
+        } else {
+            if (l < ncols) out[(int64_t)e * A.out_es + l] = v;
+        }
     };
     if constexpr (MODE == 0) {
         tile_fft_io<T, N, L, L, WG, S, 0, false, false>(lds, A.tw, gload, gstore);
@@ -321,7 +329,13 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
             const T sx = sizeof(T) == 4 ? (T)A.green_sf[kx] : (T)A.green_s[kx];
             const T sy = sizeof(T) == 4 ? (T)A.green_sf[e] : (T)A.green_s[e];
             const T k2 = sx + sy + skz;
-            const T g = (k2 > T(0)) ? -gsc / k2 : T(0);
+            // FP32: the hardware reciprocal (within 1 ulp) instead of a correctly rounded
+            // division, 7 % of the pass; FP64 divides
+            T g = T(0);
+            if (k2 > T(0)) {
+                if constexpr (sizeof(T) == 4) g = -gsc * __builtin_amdgcn_rcpf(k2);
+                else g = -gsc / k2;
+            }
             v.x *= g;
             v.y *= g;
             lds[e * L + l] = v;
@@ -647,16 +661,23 @@ static int launch_strided(const StridedArgs<T> &A, int S, int mode, int nbatch, 
     constexpr int L = Tune<T>::template strided_L<N>();
     constexpr int LG = Tune<T>::template green_L<N>();
     LF2MI_PROF(mode == 1 ? "fft_y_green" : (S < 0 ? "fft_strided_fwd" : "fft_strided_inv"), st);
+    const int64_t es = std::max(A.in_es, A.out_es);
+    const bool i32 = (int64_t)(N - 1) * es + std::max(L, LG) < ((int64_t)1 << 31);
     if (mode == 1) {
         dim3 grid((A.ncols + LG - 1) / LG, nbatch);
-        fft_strided_kernel<T, N, LG, WG, -1, 1><<<grid, WG, sizeof(cplx<T>) * N * LG, st>>>(A);
+        const size_t shm = sizeof(cplx<T>) * N * LG;
+        if (i32) fft_strided_kernel<T, N, LG, WG, -1, 1, true><<<grid, WG, shm, st>>>(A);
+        else fft_strided_kernel<T, N, LG, WG, -1, 1, false><<<grid, WG, shm, st>>>(A);
     } else {
         dim3 grid((A.ncols + L - 1) / L, nbatch);
         const size_t shm = sizeof(cplx<T>) * N * L;
-        if (S < 0)
-            fft_strided_kernel<T, N, L, WG, -1, 0><<<grid, WG, shm, st>>>(A);
-        else
-            fft_strided_kernel<T, N, L, WG, +1, 0><<<grid, WG, shm, st>>>(A);
+        if (S < 0) {
+            if (i32) fft_strided_kernel<T, N, L, WG, -1, 0, true><<<grid, WG, shm, st>>>(A);
+            else fft_strided_kernel<T, N, L, WG, -1, 0, false><<<grid, WG, shm, st>>>(A);
+        } else {
+            if (i32) fft_strided_kernel<T, N, L, WG, +1, 0, true><<<grid, WG, shm, st>>>(A);
+            else fft_strided_kernel<T, N, L, WG, +1, 0, false><<<grid, WG, shm, st>>>(A);
+        }
     }
     LF2MI_LAUNCHED();
     return LF2MI_OK;
