@@ -358,10 +358,14 @@ __global__ __launch_bounds__(WG) void fft_strided_kernel(StridedArgs<T> A) {
 // (2p, 2p+1) of component c as re/im.  Row r = y + N z of the real interior.
 // CC: the component count when it is known at compile time (1: the Poisson solve's field),
 // else 0 and A.C
+// LDS rows of a tile: L lines plus one of padding (bank conflicts), except for the six
+// component tile at N >= 1024, where a third workgroup per CU counts for more
+template <int N, int L, int CC> constexpr int x_es() { return (N >= 1024 && CC == 6) ? L : L + 1; }
+
 template <typename T, int N, int L, int WG, int CC>
 __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
     using V = cplx<T>;
-    constexpr int ES = L + 1;
+    constexpr int ES = x_es<N, L, CC>();
     extern __shared__ unsigned char smem_raw[];
     V *lds = reinterpret_cast<V *>(smem_raw);
     const int C = CC ? CC : A.C;
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(WG) void fft_x_r2c_kernel(XArgs<T> A) {
 template <typename T, int N, int L, int WG, int CC>
 __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
     using V = cplx<T>;
-    constexpr int ES = L + 1;
+    constexpr int ES = x_es<N, L, CC>();
     extern __shared__ unsigned char smem_raw[];
     V *lds = reinterpret_cast<V *>(smem_raw);
     const int C = CC ? CC : A.C;
@@ -629,6 +633,9 @@ template <> struct Tune<float> {
     // the field has more components than the narrow tile holds
     template <int N> static constexpr int x_L() { return N >= 512 ? 4 : (N >= 128 ? 12 : 24); }
     template <int N> static constexpr int x_L_max() { return N >= 512 ? 6 : (N >= 128 ? 12 : 24); }
+    // x pass workgroup: 512 threads at N = 1024 (1024^3 forward / backward x passes 2.87 / 2.63
+    // -> 2.41 / 2.35 ms for one component, 21.6 / 23.1 -> 20.6 / 20.9 ms for six)
+    template <int N> static constexpr int x_WG() { return N >= 1024 ? 512 : 256; }
     // C2C z pass: 128-byte column reads (padded LDS tile N x (L+1), up to 136 KB at 1024)
     template <int N> static constexpr int c2c_z_L() { return 16; }
     // C2C x pass: lines per tile (8 at N >= 512: 1.65 -> 1.55 ms forward at 512^3)
@@ -640,6 +647,7 @@ template <> struct Tune<double> {
     template <int N> static constexpr int green_L() { return N >= 1024 ? 4 : 8; }
     template <int N> static constexpr int x_L() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
     template <int N> static constexpr int x_L_max() { return N >= 512 ? 3 : (N >= 128 ? 6 : 12); }
+    template <int N> static constexpr int x_WG() { return 256; }
     template <int N> static constexpr int c2c_z_L() { return 8; }
     template <int N> static constexpr int c2c_x_L() { return x_L_max<N>(); }
 };
@@ -692,11 +700,12 @@ static int launch_strided(const StridedArgs<T> &A, int S, int mode, int nbatch, 
 
 template <typename T, int N, int L>
 static int launch_x_tile(const XArgs<T> &A, int S, hipStream_t st) {
-    constexpr int WG = Tune<T>::WG;
+    constexpr int WG = Tune<T>::template x_WG<N>();
     const int P = L / A.C;
     const int64_t npairs = (int64_t)N * N / 2;
     const unsigned grid = (unsigned)((npairs + P - 1) / P);
-    const size_t shm = sizeof(cplx<T>) * N * (L + 1);
+    const bool c6 = A.C == 6 && L >= 6;  // the CC = 6 kernels below
+    const size_t shm = sizeof(cplx<T>) * N * (c6 ? x_es<N, L, 6>() : x_es<N, L, 1>());
     LF2MI_PROF(S < 0 ? "fft_x_r2c" : "fft_x_c2r", st);
     // the component counts of the PM step (1) and of vector (3) and symmetric tensor (6) fields
     // are compile-time constants, so that the tile's line -> (pair, component) splits are
