@@ -626,8 +626,10 @@ __global__ __launch_bounds__(WG) void fft_z_c2c_kernel(C2CArgs<T> A) {
 template <typename T> struct Tune;
 template <> struct Tune<float> {
     static constexpr int WG = 256;
-    // 16 columns (128-byte rows); at N = 1024 the tile is 128 KB of the CU's 160 KB of LDS
-    template <int N> static constexpr int strided_L() { return 16; }
+    // 16 columns (128-byte rows); at N = 1024 8 columns (64 KB tiles, two workgroups per CU
+    // instead of one; the 64-byte rows are remapped over the XCDs): 1024^3 z / y passes 3.1 ->
+    // 2.3 ms for one component, 16.6 -> 14.2 ms for six (4 columns: 2.8 / 17.1 ms)
+    template <int N> static constexpr int strided_L() { return N >= 1024 ? 8 : 16; }
     template <int N> static constexpr int green_L() { return N >= 512 ? 8 : 16; }
     // x pass: lines per tile for one component (row pairs), and the widest tile, used when
     // the field has more components than the narrow tile holds
