@@ -1541,8 +1541,11 @@ __global__ __launch_bounds__(DEP_WG) void deposit_dense_fold_kernel(DepArgs A, c
                                                                     const ACC *__restrict__ zsh,
                                                                     const ACC *__restrict__ xsh, int fold_blocks) {
     static_assert(DEP_WG == 256, "workgroup size of both parts");
-    if ((int)blockIdx.x < fold_blocks) {
-        const unsigned n = fold_nodes(A.g.N, A.g.NB), stride = (unsigned)fold_blocks * DEP_WG;
+    // without dense bricks (an even spread of particles) every workgroup folds
+    const bool all_fold = ngroups[0] == 0;
+    if ((int)blockIdx.x < fold_blocks || all_fold) {
+        const unsigned n = fold_nodes(A.g.N, A.g.NB);
+        const unsigned stride = (unsigned)(all_fold ? (int)gridDim.x : fold_blocks) * DEP_WG;
         for (unsigned t = blockIdx.x * DEP_WG + threadIdx.x; t < n; t += stride)
             fold_node<ACC>(A, ysh, zsh, xsh, t);
         return;
@@ -1950,24 +1953,6 @@ __device__ __forceinline__ float tc_seam_sum(const TcFold &F, const float *__res
     return s;
 }
 
-// the contributions of every component to node (X, Y, Z), added to the field
-__device__ __forceinline__ void tc_fold_node(float *__restrict__ field, int N, int h, int NB, const TcFold &F,
-                                             const float *__restrict__ shell, int X, int Y, int Z) {
-    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    bool any = false;
-    for (int c = 0; c < F.C; ++c) {
-        int64_t off[8];
-        const int n = tc_row_sources(F, N, NB, c, Y, Z, off);
-        float s = tc_seam_sum(F, shell, NB, c, X, Y, Z);
-        for (int k = 0; k < n; ++k) s += shell[off[k] + X];
-        v[c] = s;
-        any |= s != 0.0f;
-    }
-    if (!any) return;
-    float *f = field + site_index(N, h, X, Y, Z) * F.C;
-    for (int c = 0; c < F.C; ++c) f[c] += v[c];
-}
-
 // the receiving nodes of whole rows along x: the sy layers (every z), then the cy layers
 // with the sz layers; one workgroup per row.  The row's shell sources are found once and
 // summed for every x (coalesced); then the row's tile-edge nodes add their seam values, one
@@ -2032,6 +2017,7 @@ __global__ __launch_bounds__(256) void fold_rows_kernel(float *__restrict__ fiel
 }
 
 // the remaining receiving nodes: tile edges (sx) in the cy layers and cz layers
+template <int C>
 __global__ __launch_bounds__(256) void fold_edges_kernel(float *__restrict__ field, int N, int h, int NB, TcFold F,
                                                          const float *__restrict__ shell, int64_t total) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -2043,7 +2029,18 @@ __global__ __launch_bounds__(256) void fold_edges_kernel(float *__restrict__ fie
     const int X = (xi / F.nx) * F.TX + F.sx[xi % F.nx];
     const int Y = (yi / (8 - F.ny)) * 8 + F.cy[yi % (8 - F.ny)];
     const int Z = (zi / (8 - F.nz)) * 8 + F.cz[zi % (8 - F.nz)];
-    tc_fold_node(field, N, h, NB, F, shell, X, Y, Z);
+    // no shell row lands on these y and z layers: the seams only
+    float v[C];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        v[c] = tc_seam_sum(F, shell, NB, c, X, Y, Z);
+        any |= v[c] != 0.0f;
+    }
+    if (!any) return;
+    float *f = field + site_index(N, h, X, Y, Z) * C;
+#pragma unroll
+    for (int c = 0; c < C; ++c) f[c] += v[c];
 }
 
 // maxima over the particles of |q| (out[0]) and |v_d| (out[1 + d]) as the bits of
@@ -3111,7 +3108,7 @@ static int launch_tiles_c(lf2mi_parts_s *p, DepArgs A, bool pm, bool overwrite, 
     }
     const int64_t edges = (int64_t)F.nx * ntile * (8 - F.ny) * NB * (8 - F.nz) * NB;
     if (edges > 0) {
-        fold_edges_kernel<<<grid1(edges), 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell, edges);
+        fold_edges_kernel<C><<<grid1(edges), 256, 0, st>>>(A.field, (int)N, A.h, (int)NB, F, shell, edges);
         LF2MI_LAUNCHED();
     }
     if (int rc = fill_int(p->nditem, 2, 0, st)) return rc;
