@@ -174,7 +174,8 @@ def deposit_kinds(args):
     N(0, 0.01), mass 1/N^3: the large multi-component workload.  One JSON line per kind
     (the scalar kind with per-particle masses as "scalar_per_particle_mass"): ms per deposit
     (the mean of --steps deposits after --warmup, HIP events on the launch stream) and the
-    per-kernel split."""
+    per-kernel split; for the scalar, vector and VecVec kinds also the R2C forward and C2R
+    backward transforms of the deposited field (ms per transform, and per pass launch)."""
     import torch
 
     import latfield2_amd as lf
@@ -214,10 +215,30 @@ def deposit_kinds(args):
             pb = 16 + (12 if kind != lf.PROJ_SCALAR else 0) + (4 if pm else 0)
             ab = N ** 3 * pb + 2 * N ** 3 * C * 4
             name = names[kind] + ("_per_particle_mass" if pm else "")
-            print(json.dumps({"deposit": name, "mesh": N, "particles": N ** 3, "components": C,
-                              "ms": round(ms, 3), "algorithmic_GB": round(ab / 1e9, 2),
-                              "GBps": round(ab / (ms * 1e-3) / 1e9, 1),
-                              "stages": {k: round(t / args.steps, 3) for k, (t, c) in pr.items()}}), flush=True)
+            line = {"deposit": name, "mesh": N, "particles": N ** 3, "components": C,
+                    "ms": round(ms, 3), "algorithmic_GB": round(ab / 1e9, 2),
+                    "GBps": round(ab / (ms * 1e-3) / 1e9, 1),
+                    "stages": {k: round(t / args.steps, 3) for k, (t, c) in pr.items()}}
+            if not pm and kind in (lf.PROJ_SCALAR, lf.PROJ_VECTOR, lf.PROJ_VECVEC):
+                # the R2C forward and C2R backward transforms of the deposited field (every
+                # component), each pass reading and writing its data once
+                k = lf.KField(lat, 0, C, lf.F32)
+                plan = lf.PlanFFT(f, k)
+                plan.execute(lf.FFT_FORWARD)
+                plan.execute(lf.FFT_BACKWARD)
+                for d, tag in ((lf.FFT_FORWARD, "fft_forward"), (lf.FFT_BACKWARD, "fft_backward")):
+                    torch.cuda.synchronize()
+                    _lib.profile_reset()
+                    _lib.profile_enable(True)
+                    for _ in range(args.steps):
+                        plan.execute(d)
+                    torch.cuda.synchronize()
+                    _lib.profile_enable(False)
+                    pr = _lib.profile_read()
+                    line[tag + "_ms"] = round(sum(t for t, _ in pr.values()) / args.steps, 3)
+                    line[tag + "_passes"] = {kk: round(t / c, 3) for kk, (t, c) in pr.items()}
+                del plan, k
+            print(json.dumps(line), flush=True)
             del f
             torch.cuda.empty_cache()
         del parts
