@@ -2447,14 +2447,19 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
         const int ox = bx0 * B, oy = by * B, oz = bz * B;
         const int wx = nbx * B + 3, wy = B + 3, nh = (nbx + 2) * 9;
         for (int i = threadIdx.x; i < nh; i += DEP_WG) hist[i] = 0;
-        // the first batch of particle data in flight while the window loads
-        int p = ch.x + threadIdx.x;
-        int cell = 0;
-        float u[3] = {0.f, 0.f, 0.f}, vin[3] = {0.f, 0.f, 0.f};
-        if (p < ch.y) {
-            cell = A.s.cell[p];
-            u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
-            vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
+        // every particle of the chunk (at most ITEM_P) in flight while the window loads
+        constexpr int CP = ITEM_P / DEP_WG;
+        int pcell[CP];
+        float pu[CP][3], pv[CP][3];
+#pragma unroll
+        for (int k = 0; k < CP; ++k) {
+            const int q = ch.x + threadIdx.x + k * DEP_WG;
+            pcell[k] = 0;
+            if (q < ch.y) {
+                pcell[k] = A.s.cell[q];
+                pu[k][0] = A.s.off[0][q]; pu[k][1] = A.s.off[1][q]; pu[k][2] = A.s.off[2][q];
+                pv[k][0] = A.s.vel[0][q]; pv[k][1] = A.s.vel[1][q]; pv[k][2] = A.s.vel[2][q];
+            }
         }
         // rows (y, z) of the window, one wave per row: the row's base index is wave-uniform,
         // a lane holds x = lane and x = lane + 64; every row's loads are issued before the
@@ -2494,12 +2499,12 @@ __global__ __launch_bounds__(DEP_WG) void gkd_chunk_kernel(GkdArgs A, const int4
         }
         __syncthreads();
         if (threadIdx.x == 0) claim = atomicAdd(&nchunk[1], 1);  // every lane has read ci
-        for (; p < ch.y; p += DEP_WG) {
-            if (p != ch.x + threadIdx.x) {
-                cell = A.s.cell[p];
-                u[0] = A.s.off[0][p]; u[1] = A.s.off[1][p]; u[2] = A.s.off[2][p];
-                vin[0] = A.s.vel[0][p]; vin[1] = A.s.vel[1][p]; vin[2] = A.s.vel[2][p];
-            }
+#pragma unroll
+        for (int k = 0; k < CP; ++k) {
+            const int p = ch.x + threadIdx.x + k * DEP_WG;
+            if (p >= ch.y) break;
+            const int cell = pcell[k];
+            float u[3] = {pu[k][0], pu[k][1], pu[k][2]}, vin[3] = {pv[k][0], pv[k][1], pv[k][2]};
             int cx, cy, cz;
             decode_cell(cell, A.g, cx, cy, cz);
             // window coordinates: cell c sits at c - o + 1

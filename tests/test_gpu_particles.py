@@ -185,8 +185,9 @@ def test_cic_ngp_deposits_match_oracle(prec, tol, N, h, n, clustered, masses, mo
 @pytest.mark.parametrize("N,n,clustered,masses", [
     (64, 360000, False, False),   # ~700 per brick: most of them read again in every component pass
     (64, 200000, True, True),     # a blob of dense bricks (item kernel) among sparse ones
-    (128, 400000, True, False),   # two tiles per row of bricks: seams between tiles and across the edge
-    (128, 400000, True, True)])
+    (128, 400000, True, False),   # several tiles per row of bricks: seams between tiles and across the edge
+    (128, 400000, True, True),
+    (96, 300000, True, True)])    # N not a power of two (12 bricks per row: 3 tiles of the multi-component kernel)
 def test_periodic_tile_deposits_match_oracle(N, n, clustered, masses):
     """the periodic FP32 deposits on lattices whose rows of bricks split into tiles of 8:
     every kind (and the scalar kind with per-particle masses) against the FP64 oracle"""
