@@ -456,30 +456,45 @@ __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
     const int HC = H * C;
     // gather rows a, b into Z: Z[k] = A + iB (k in [0, N/2]), Z[N-k] = conj A + i conj B
     if constexpr (CC == 1) {
-        // one row pair at a time (its base addresses uniform), threads along kx
-        for (int p = 0; p < L; ++p) {
+        // the (row pair, kx) items spread over the threads, kx fastest, every item's two
+        // loads issued before any is used (one memory round trip for the tile instead of one
+        // per row pair: 512^3 0.305 -> 0.299 ms, 1024^3 2.35 -> 2.07 ms)
+        constexpr int NI = (L * H + WG - 1) / WG;
+        V va[NI], vb[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * WG;
+            const int p = i / H, kx = i - p * H;
             const int64_t pr = pair0 + p;
-            if (pr >= npairs) break;
-            const int64_t ra = 2 * pr;
-            const int ya = (int)(ra % N), z = (int)(ra / N);
-            const V *srca = A.kin + A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)ya * A.k_s2;
-            const V *srcb = srca + A.k_s2;  // row ya + 1 of the same plane (N even)
-            for (int kx = threadIdx.x; kx < H; kx += WG) {
-                V a = srca[kx], b = srcb[kx];
-                if (kx == 0 || kx == N / 2) {
-                    a.y = 0;
-                    b.y = 0;
-                }
-                V z1;
-                z1.x = a.x - b.y;
-                z1.y = a.y + b.x;
-                lds[kx * ES + p] = z1;
-                if (kx != 0 && kx != N / 2) {
-                    V z2;
-                    z2.x = a.x + b.y;
-                    z2.y = b.x - a.y;
-                    lds[(N - kx) * ES + p] = z2;
-                }
+            va[k] = mk<T>(0, 0);
+            vb[k] = mk<T>(0, 0);
+            if (i < L * H && pr < npairs) {
+                const int64_t ra = 2 * pr;
+                const int ya = (int)(ra % N), z = (int)(ra / N);
+                const V *srca = A.kin + A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)ya * A.k_s2;
+                va[k] = srca[kx];
+                vb[k] = srca[A.k_s2 + kx];  // row ya + 1 of the same plane (N even)
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * WG;
+            const int p = i / H, kx = i - p * H;
+            if (i >= L * H || pair0 + p >= npairs) continue;
+            V a = va[k], b = vb[k];
+            if (kx == 0 || kx == N / 2) {
+                a.y = 0;
+                b.y = 0;
+            }
+            V z1;
+            z1.x = a.x - b.y;
+            z1.y = a.y + b.x;
+            lds[kx * ES + p] = z1;
+            if (kx != 0 && kx != N / 2) {
+                V z2;
+                z2.x = a.x + b.y;
+                z2.y = b.x - a.y;
+                lds[(N - kx) * ES + p] = z2;
             }
         }
     } else
