@@ -497,6 +497,48 @@ __global__ __launch_bounds__(WG) void fft_x_c2r_kernel(XArgs<T> A) {
                 lds[(N - kx) * ES + p] = z2;
             }
         }
+    } else if constexpr (CC > 1) {
+        // the same with the component count known: all of the tile's loads in flight at once
+        constexpr int PC = L / CC, HCC = H * CC, NI = (PC * HCC + WG - 1) / WG;
+        V va[NI], vb[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * WG;
+            const int p = i / HCC, q = i - p * HCC;
+            const int64_t pr = pair0 + p;
+            va[k] = mk<T>(0, 0);
+            vb[k] = mk<T>(0, 0);
+            if (i < PC * HCC && pr < npairs) {
+                const int64_t ra = 2 * pr;
+                const int ya = (int)(ra % N), z = (int)(ra / N);
+                const V *srca = A.kin + A.k_col0 + (int64_t)z * A.k_s1 + (int64_t)ya * A.k_s2;
+                va[k] = srca[q];
+                vb[k] = srca[A.k_s2 + q];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * WG;
+            const int p = i / HCC, q = i - p * HCC;
+            if (i >= PC * HCC || pair0 + p >= npairs) continue;
+            const int kx = q / CC, c = q - kx * CC;
+            const int l = p * CC + c;
+            V a = va[k], b = vb[k];
+            if (kx == 0 || kx == N / 2) {
+                a.y = 0;
+                b.y = 0;
+            }
+            V z1;
+            z1.x = a.x - b.y;
+            z1.y = a.y + b.x;
+            lds[kx * ES + l] = z1;
+            if (kx != 0 && kx != N / 2) {
+                V z2;
+                z2.x = a.x + b.y;
+                z2.y = b.x - a.y;
+                lds[(N - kx) * ES + l] = z2;
+            }
+        }
     } else
     for (int i = threadIdx.x; i < P * HC; i += WG) {
         const int q = i % HC;
