@@ -219,6 +219,21 @@ def deposit_kinds(args):
                     "ms": round(ms, 3), "algorithmic_GB": round(ab / 1e9, 2),
                     "GBps": round(ab / (ms * 1e-3) / 1e9, 1),
                     "stages": {k: round(t / args.steps, 3) for k, (t, c) in pr.items()}}
+            # the same deposit replacing the field (projection_init + _project + _comm in
+            # one call, LF2MI_DEP_PERIODIC_SET): the interior is stored, not read back
+            torch.cuda.synchronize()
+            _lib.profile_reset()
+            _lib.profile_enable(True)
+            for _ in range(args.steps):
+                parts.deposit(kind, f, lf.DEP_PERIODIC_SET, lf.ACC_NATIVE)
+            torch.cuda.synchronize()
+            _lib.profile_enable(False)
+            pr = _lib.profile_read()
+            ms_set = sum(t for t, _ in pr.values()) / args.steps
+            ab_set = N ** 3 * pb + N ** 3 * C * 4
+            line["set_ms"] = round(ms_set, 3)
+            line["set_GBps"] = round(ab_set / (ms_set * 1e-3) / 1e9, 1)
+            line["set_stages"] = {k: round(t / args.steps, 3) for k, (t, c) in pr.items()}
             if not pm and kind in (lf.PROJ_SCALAR, lf.PROJ_VECTOR, lf.PROJ_VECVEC):
                 # the R2C forward and C2R backward transforms of the deposited field (every
                 # component), each pass reading and writing its data once

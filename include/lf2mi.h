@@ -55,6 +55,10 @@ extern "C" {
 /* deposit target */
 #define LF2MI_DEP_UNFOLDED 0 /* like *_project: adds into interior and ghost nodes */
 #define LF2MI_DEP_PERIODIC 1 /* like *_project then *_comm: adds into interior nodes */
+/* projection_init, *_project and *_comm in one call: the interior receives the periodic
+ * deposit, the ghosts zero (no counterpart in the reference, which zeroes the whole field
+ * first: the FP32 deposits then store the interior instead of reading it back) */
+#define LF2MI_DEP_PERIODIC_SET 2
 
 /* deposit accumulation */
 #define LF2MI_ACC_NATIVE 0  /* atomics in the container's precision */
@@ -316,7 +320,8 @@ int lf2mi_parts_ic_blobs(lf2mi_parts p, int64_t np, int nblobs, double alpha,
  *         depend on the order of the additions (bitwise reproducible); scalar kind: 2^-32 of
  *         the mass per unit weight; vector and tensor kinds: per component 2^(e-62) with
  *         np max|q| max|v_a| max|v_b| < 2^e (finding the maxima synchronises the stream).
- * Values are added (+=): zero the field first, as projection_init does. */
+ * Values are added (+=): zero the field first, as projection_init does, or ask for
+ * LF2MI_DEP_PERIODIC_SET, which replaces the field's contents. */
 int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc,
                   void *stream);
 

@@ -20,8 +20,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_UNFOLDED, F32, F64, MASS_INFO, MASS_PART, MASS_PART_F32,
-                   FFT_BACKWARD, FFT_FORWARD, MAX, MAX_LOCAL, MIN, MIN_LOCAL, PARTS_ID, PARTS_MASS, PQ_EKIN,
+from ._lib import (ACC_FIXED64, ACC_NATIVE, DEP_PERIODIC, DEP_PERIODIC_SET, DEP_UNFOLDED, F32, F64, MASS_INFO,
+                   MASS_PART, MASS_PART_F32, FFT_BACKWARD, FFT_FORWARD, MAX, MAX_LOCAL, MIN, MIN_LOCAL, PARTS_ID, PARTS_MASS, PQ_EKIN,
                    PQ_MASS, PQ_MVX, PQ_MVY, PQ_MVZ, PQ_V2, PROJ_SCALAR, PROJ_SYMTENSOR_NGP, PROJ_VECTOR,
                    PROJ_VECTOR_NGP, PROJ_VECVEC, SUM, SUM_LOCAL, Lf2miError, call, load)
 

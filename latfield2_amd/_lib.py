@@ -16,7 +16,7 @@ F64 = 1
 FFT_FORWARD = 1
 FFT_BACKWARD = -1
 PROJ_SCALAR, PROJ_VECTOR, PROJ_VECVEC, PROJ_VECTOR_NGP, PROJ_SYMTENSOR_NGP = 0, 1, 2, 3, 4
-DEP_UNFOLDED, DEP_PERIODIC = 0, 1
+DEP_UNFOLDED, DEP_PERIODIC, DEP_PERIODIC_SET = 0, 1, 2
 ACC_NATIVE, ACC_FIXED64 = 0, 1
 PARTS_ID, PARTS_MASS = 1, 2
 MASS_INFO, MASS_PART, MASS_PART_F32 = 0, 1, 2

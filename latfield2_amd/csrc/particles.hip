@@ -3844,9 +3844,28 @@ static int deposit_impl(lf2mi_parts p, int kind, void *field, int halo, int targ
     if (halo < 1) return fail(LF2MI_EINVAL, "the field has to have at least a halo of 1");
     if (target != LF2MI_DEP_UNFOLDED && target != LF2MI_DEP_PERIODIC) return fail(LF2MI_EINVAL, "bad target %d", target);
     if (acc != LF2MI_ACC_NATIVE && acc != LF2MI_ACC_FIXED64) return fail(LF2MI_EINVAL, "bad accumulation mode %d", acc);
+    if (overwrite && target != LF2MI_DEP_PERIODIC) return fail(LF2MI_EINVAL, "an overwriting deposit is periodic");
     hipStream_t st = as_stream(stream);
     const int N = p->N;
     const double nv = (double)N * N * N;
+    if (overwrite) {
+        // the field's previous contents are replaced: the FP32 tiles kernels store every
+        // interior node, so only the ghosts need clearing; the other paths add into a
+        // cleared field
+        static const int comps[5] = {1, 3, 6, 3, 6};
+        const int C = comps[kind];
+        const int64_t G = N + 2 * halo;
+        const bool f32 = p->dtype == LF2MI_F32;
+        const bool pmass = p->mass_src && (f32 ? p->b32[p->cur].mass != nullptr : p->b64.mass != nullptr);
+        const bool stores = f32 && acc == LF2MI_ACC_NATIVE && p->np > 0 &&
+                            (tiles_path(p, kind, target, pmass) || tiles_c_path(p, kind, target, pmass));
+        if (stores) {
+            if (int rc = zero_ghosts(field, N, halo, C, p->dtype, st)) return rc;
+        } else {
+            if (int rc = lf2mi_zero(field, G * G * G * C, p->dtype, stream)) return rc;
+            overwrite = false;
+        }
+    }
     if (p->np == 0) return LF2MI_OK;
     if (p->dtype == LF2MI_F64) {
         if (acc != LF2MI_ACC_NATIVE) return fail(LF2MI_EUNSUPPORTED, "fixed-point deposit is an F32 option");
@@ -4007,6 +4026,8 @@ static int deposit_impl(lf2mi_parts p, int kind, void *field, int halo, int targ
 extern "C" {
 
 int lf2mi_deposit(lf2mi_parts p, int kind, void *field, int halo, int target, int acc, void *stream) {
+    if (target == LF2MI_DEP_PERIODIC_SET)
+        return deposit_impl(p, kind, field, halo, LF2MI_DEP_PERIODIC, acc, true, stream);
     return deposit_impl(p, kind, field, halo, target, acc, false, stream);
 }
 
@@ -4143,16 +4164,8 @@ int lf2mi_pm_step(lf2mi_parts p, lf2mi_fft_plan plan, void *rho, void *phi, int 
                   int acc, void *stream) {
     if (!p || !plan || !rho || !phi) return fail(LF2MI_EINVAL, "lf2mi_pm_step: null argument");
     const int N = p->N;
-    const int64_t G = N + 2 * halo;
-    // the native tiles-path deposit stores every interior node: only the ghosts need clearing
-    const bool store = p->dtype == LF2MI_F32 && acc == LF2MI_ACC_NATIVE && halo >= 1 &&
-                       tiles_path(p, LF2MI_PROJ_SCALAR, LF2MI_DEP_PERIODIC, p->mass_src && p->b32[p->cur].mass);
-    if (store) {
-        if (int rc = zero_ghosts(rho, N, halo, 1, p->dtype, as_stream(stream))) return rc;
-    } else if (int rc = lf2mi_zero(rho, G * G * G, p->dtype, stream)) {
-        return rc;
-    }
-    if (int rc = deposit_impl(p, LF2MI_PROJ_SCALAR, rho, halo, LF2MI_DEP_PERIODIC, acc, store, stream)) return rc;
+    // rho replaced by the deposit (the FP32 tiles kernels store its interior)
+    if (int rc = deposit_impl(p, LF2MI_PROJ_SCALAR, rho, halo, LF2MI_DEP_PERIODIC, acc, true, stream)) return rc;
     // box volume 1: the mean density is the total mass
     const double scale = 1.0 / ((double)N * N * N);
     if (int rc = lf2mi_poisson_solve(plan, rho, phi, p->total_mass, scale, stream)) return rc;

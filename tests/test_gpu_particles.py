@@ -226,6 +226,46 @@ def test_periodic_tile_deposits_match_oracle(N, n, clustered, masses):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("N,n,masses", [(64, 200000, True), (128, 400000, False), (32, 40000, False)])
+def test_periodic_set_deposit_replaces_the_field(prec, N, n, masses):
+    """LF2MI_DEP_PERIODIC_SET == projection_init + periodic deposit, whatever the field held:
+    the interior equals the deposit into a cleared field (bitwise with the fixed point, FP32
+    atomics to rounding), the ghosts are zero"""
+    announce(f"periodic set deposit {prec} N={N} n={n} masses={masses}")
+    pos, vel, mass = _random_particles(n, 5 * N + n, clustered=True)
+    parts, _ = _upload(N, pos, vel, prec, mass if masses else None)
+    h = 1
+    lat = lf.Lattice(3, N, h)
+    accs = [lf.ACC_NATIVE] + ([lf.ACC_FIXED64] if prec == "f32" else [])
+    for kind, C in ((lf.PROJ_SCALAR, 1), (lf.PROJ_VECTOR, 3), (lf.PROJ_VECVEC, 6), (lf.PROJ_VECTOR_NGP, 3),
+                    (lf.PROJ_SYMTENSOR_NGP, 6)):
+        for acc in accs:
+            a, b = lf.Field(lat, C, DT[prec]), lf.Field(lat, C, DT[prec])
+            a.t.fill_(0.25)
+            parts.deposit(kind, a, lf.DEP_PERIODIC_SET, acc)
+            lf.projection_init(b)
+            parts.deposit(kind, b, lf.DEP_PERIODIC, acc)
+            torch.cuda.synchronize()
+            G = N + 2 * h
+            ga, gb = real_np(a).reshape(G, G, G, C), real_np(b).reshape(G, G, G, C)
+            s = slice(h, N + h)
+            ghosts = ga.copy()
+            ghosts[s, s, s] = 0.0
+            assert not ghosts.any(), (kind, acc)
+            if acc == lf.ACC_FIXED64:
+                assert np.array_equal(ga, gb), kind
+            else:
+                err = rel_l2(ga, gb)
+                assert err <= (1e-12 if prec == "f64" else 1e-6), (kind, acc, err)
+    empty = lf.Particles(N, 10, DT[prec])
+    f = lf.Field(lat, 3, DT[prec])
+    f.t.fill_(0.25)
+    empty.deposit(lf.PROJ_VECTOR, f, lf.DEP_PERIODIC_SET)
+    torch.cuda.synchronize()
+    assert not bool(f.t.any())
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("N", [16, 64])
 def test_deposits_of_an_empty_container_add_nothing(prec, N):
     announce(f"empty container deposits {prec} N={N}")
